@@ -1,0 +1,178 @@
+"""fmma_amd -- a black-box FMM for AMD MI355X (gfx950).
+
+Computes c_i = sum_j w_j f(x_i, y_j) with the Chebyshev black-box FMM of
+fockl/FMMA on the GPU.  ``Plan`` mirrors ``fmma::FMMA<TYPE,DIM>``
+(include/fmma/fmma.hpp:52-170 of the reference) over the C-ABI of
+``libfmma_hip.so``; ``fmma_amd.pyfmma`` is the drop-in for the reference's
+``pyfmma`` module.  Arrays may be torch tensors on the GPU (device pointers,
+solved in place on torch's current stream) or numpy arrays (staged through
+device memory).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FMMAError, KERNELS, M2L_PRECISION  # noqa: F401
+
+__all__ = ["Plan", "FMMAError", "KERNELS", "M2L_PRECISION", "version"]
+
+
+def version():
+    return _lib.lib().fmma_version().decode()
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class Plan:
+    """One FMM problem configuration: ``fmma::FMMA<TYPE, DIM>``.
+
+    The defaults are the reference's (include/fmma/fmma.hpp:90-127): solver
+    "exact", poly_ord 1, Depth -1 (auto, sticky), trans_sym_flag 0, kernel
+    1/|x-y|.
+    """
+
+    def __init__(self, dim, dtype="float64"):
+        self.dim = int(dim)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float64, np.float32):
+            raise ValueError("dtype must be float64 or float32")
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(L.fmma_plan_create(ctypes.byref(h), self.dim,
+                                      _lib.FMMA_F64 if self.dtype == np.float64 else _lib.FMMA_F32))
+        self._h = h
+        self._L = L
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.fmma_plan_destroy(h)
+            self._h = None
+
+    # ---- setters (include/fmma/fmma.hpp:65-127)
+    def set_solver_type(self, s):
+        _lib.check(self._L.fmma_plan_set_solver_type(self._h, s.encode()))
+
+    def set_poly_ord(self, p):
+        _lib.check(self._L.fmma_plan_set_poly_ord(self._h, int(p)))
+
+    def set_Depth(self, d):
+        _lib.check(self._L.fmma_plan_set_depth(self._h, int(d)))
+
+    set_depth = set_Depth
+
+    @property
+    def depth(self):
+        d = ctypes.c_int()
+        _lib.check(self._L.fmma_plan_get_depth(self._h, ctypes.byref(d)))
+        return d.value
+
+    def set_trans_sym_flag(self, f):
+        _lib.check(self._L.fmma_plan_set_trans_sym_flag(self._h, int(f)))
+
+    def set_nrn_N(self, n):
+        _lib.check(self._L.fmma_plan_set_nrn_N(self._h, int(n)))
+
+    def set_kernel(self, name, *params):
+        kid = KERNELS[name] if isinstance(name, str) else int(name)
+        p = (ctypes.c_double * max(1, len(params)))(*params)
+        _lib.check(self._L.fmma_plan_set_kernel(self._h, kid, p, len(params)))
+
+    def set_m2l_precision(self, prec):
+        v = M2L_PRECISION[prec] if isinstance(prec, str) else int(prec)
+        _lib.check(self._L.fmma_plan_set_m2l_precision(self._h, v))
+
+    def set_stream(self, stream_ptr):
+        _lib.check(self._L.fmma_plan_set_stream(self._h, ctypes.c_void_p(int(stream_ptr or 0))))
+
+    def set_timing(self, on):
+        _lib.check(self._L.fmma_plan_set_timing(self._h, 1 if on else 0))
+
+    # ---- solve (src/fmma.cpp:143-207)
+    def solve(self, targets, weights, sources, ans):
+        """ans <- sum_j w_j f(x_i, y_j).  weights may be None (all ones).
+
+        torch GPU tensors are used in place (device pointers, torch's current
+        stream); numpy arrays go through the host entry point.  As in the
+        reference, ans is overwritten by fmm when the leaf level is >= 2 and
+        added to otherwise."""
+        if _is_torch(targets):
+            return self._solve_torch(targets, weights, sources, ans)
+        return self._solve_numpy(targets, weights, sources, ans)
+
+    def _check_shapes(self, t, w, s, a):
+        if t.ndim != 2 or t.shape[1] != self.dim:
+            raise ValueError("targets must have shape (N, %d)" % self.dim)
+        if s.ndim != 2 or s.shape[1] != self.dim:
+            raise ValueError("sources must have shape (M, %d)" % self.dim)
+        if w is not None and (w.ndim != 1 or w.shape[0] != s.shape[0]):
+            raise ValueError("weights must have shape (M,)")
+        if a.ndim != 1 or a.shape[0] != t.shape[0]:
+            raise ValueError("ans must have shape (N,)")
+
+    def _solve_torch(self, t, w, s, a):
+        import torch
+        want = torch.float64 if self.dtype == np.float64 else torch.float32
+        for x in (t, w, s, a):
+            if x is None:
+                continue
+            if not x.is_cuda or x.dtype != want or not x.is_contiguous():
+                raise ValueError("torch inputs must be contiguous %s tensors on the GPU" % want)
+        self._check_shapes(t, w, s, a)
+        self.set_stream(torch.cuda.current_stream(a.device).cuda_stream)
+        _lib.check(self._L.fmma_plan_solve(self._h, t.shape[0], t.data_ptr(), w.data_ptr() if w is not None else None,
+                                           s.shape[0], s.data_ptr(), a.data_ptr()))
+        return a
+
+    def _solve_numpy(self, t, w, s, a):
+        for x in (t, w, s, a):
+            if x is None:
+                continue
+            if not isinstance(x, np.ndarray) or x.dtype != self.dtype or not x.flags.c_contiguous:
+                raise ValueError("numpy inputs must be C-contiguous %s arrays" % self.dtype)
+        if not a.flags.writeable:
+            raise ValueError("ans must be writeable")
+        self._check_shapes(t, w, s, a)
+        self.set_stream(0)
+        ptr = lambda x: x.ctypes.data if x is not None else None  # noqa: E731
+        _lib.check(self._L.fmma_plan_solve_host(self._h, t.shape[0], ptr(t), ptr(w), s.shape[0], ptr(s), ptr(a)))
+        return a
+
+    # ---- diagnostics
+    def time_log(self):
+        n = self._L.fmma_plan_time_log(self._h, None, None, 0)
+        names = (ctypes.c_char_p * max(n, 1))()
+        ms = (ctypes.c_double * max(n, 1))()
+        self._L.fmma_plan_time_log(self._h, names, ms, n)
+        return {names[i].decode(): ms[i] for i in range(n)}
+
+    def clear_time_log(self):
+        _lib.check(self._L.fmma_plan_clear_time_log(self._h))
+
+    def point_boxes(self, which, n):
+        """Row-major leaf box id of every source ('s') or target ('t') of the
+        last fmm solve, in the caller's point order."""
+        out = np.empty(int(n), dtype=np.int64)
+        _lib.check(self._L.fmma_plan_point_boxes(self._h, 1 if which == "t" else 0, int(n),
+                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_long))))
+        return out
+
+    def level_coeffs(self, which, level, rowmajor=True):
+        """Host copy of a level's multipoles (which='Wm') or locals ('Wl')
+        from the last fmm solve, [boxes, P], in the reference's row-major box
+        order by default."""
+        ptr, nb, P = ctypes.c_void_p(), ctypes.c_long(), ctypes.c_int()
+        _lib.check(self._L.fmma_plan_level_coeffs(self._h, 0 if which == "Wm" else 1, int(level), ctypes.byref(ptr),
+                                                  ctypes.byref(nb), ctypes.byref(P)))
+        out = np.empty((nb.value, P.value), dtype=self.dtype)
+        _lib.hip().hipDeviceSynchronize()
+        _lib.memcpy_d2h(out.ctypes.data, ptr.value, out.nbytes)
+        if not rowmajor:
+            return out
+        rm = np.array([self._L.fmma_morton_to_rowmajor(self.dim, int(level), m) for m in range(nb.value)])
+        res = np.empty_like(out)
+        res[rm] = out
+        return res

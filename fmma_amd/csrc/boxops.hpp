@@ -1,0 +1,49 @@
+// boxops.hpp -- P2M / M2M / L2L / L2P entry points (boxops.hip).
+#pragma once
+#include "common.hpp"
+
+namespace fmma {
+namespace lib {
+
+template <typename T>
+struct P2MArgs {
+    const unsigned *start;  // [nbox+1] source ranges, Morton order
+    const T *pos;           // [D][n] sorted coordinates
+    const T *w;             // [n] sorted weights
+    long n;
+    const Geom<T> *geom;
+    int leaf_level;
+    const T *coef;          // [nn][nn] Chebyshev weight coefficients
+    int nn, P;
+    T *Wm;                  // [nbox][P] leaf multipoles (written for every box)
+};
+
+template <typename T>
+struct L2PArgs {
+    const unsigned *start;  // [nbox+1] target ranges
+    const T *pos;           // [D][n] sorted target coordinates
+    long n;
+    const Geom<T> *geom;
+    int leaf_level;
+    const T *coef;
+    int nn, P;
+    const T *Wl;            // [nbox][P] leaf locals
+    T *out;                 // [n] sorted order
+};
+
+template <typename T, int D>
+int p2m(const P2MArgs<T> &a, long nbox, hipStream_t st);
+
+// Wp[p] = M2M of children Wc[8p .. 8p+7] for p < nparent (overwrites Wp).
+template <typename T, int D>
+int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st);
+
+// Wc[child] += L2L of Wp[parent] for every child of every parent.
+template <typename T, int D>
+int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st);
+
+template <typename T, int D>
+int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);
+
+}  // namespace lib
+}  // namespace fmma

@@ -1,0 +1,146 @@
+// builtin_kernels.hpp -- device functors for the FMMA_KERNEL_* built-ins.
+//
+// Each follows the formula of the same id in oracle/fmm_oracle.c and
+// include/fmma_hip.h; 1/r uses a hardware reciprocal square root refined by
+// one third-order Newton (Householder) step, which leaves ~1 ulp error in
+// FP64: y0 = rsq(r2), e = 1 - r2 y0^2, y = y0 (1 + e/2 + 3e^2/8).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace fmma {
+namespace lib {
+
+__device__ __forceinline__ double rsqrt_refined(double r2)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double h = r2 * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
+}
+
+__device__ __forceinline__ float rsqrt_refined(float r2)
+{
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float h = r2 * y;
+    const float e = __builtin_fmaf(-h, y, 1.0f);
+    return __builtin_fmaf(y * e, 0.5f, y);
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T dist2(const T *x, const T *y)
+{
+    T r2 = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const T t = x[d] - y[d];
+        r2 = __builtin_fma(t, t, r2);
+    }
+    return r2;
+}
+
+struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.hpp:55-63)
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        return r2 == T(0) ? T(INFINITY) : rsqrt_refined(r2);
+    }
+};
+
+struct KInvRReg {  // 1/|x-y|, 0 at r = 0
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T v = rsqrt_refined(r2);
+        return r2 == T(0) ? T(0) : v;
+    }
+};
+
+struct KInvDiff {  // 1/(x_0 - y_0)
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        return T(1) / (x[0] - y[0]);
+    }
+};
+
+struct KLogR {  // log|x-y| = 0.5 log r^2, 0 at r = 0
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        return r2 == T(0) ? T(0) : T(0.5) * std::log(r2);
+    }
+};
+
+struct KGauss {  // exp(-r^2 / (2 h^2)); p1 = 1 / (2 h^2)
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        return std::exp(-r2 * (T)p1);
+    }
+};
+
+struct KYukawa {  // exp(-kappa r) / r, 0 at r = 0; p0 = kappa
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T inv = rsqrt_refined(r2);
+        const T v = std::exp(-(T)p0 * (r2 * inv)) * inv;
+        return r2 == T(0) ? T(0) : v;
+    }
+};
+
+struct KSumR2 {  // sum_d (x_d - y_d) / r^2
+    static constexpr bool translation_invariant = true;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        T r2 = T(0), s = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T t = x[d] - y[d];
+            r2 = __builtin_fma(t, t, r2);
+            s += t;
+        }
+        return s / r2;
+    }
+};
+
+struct KSum2R2 {  // sum_d (x_d + y_d)^2 / r^2: depends on x and y, not only x - y
+    static constexpr bool translation_invariant = false;
+    double p0, p1;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        T r2 = T(0), s2 = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T t = x[d] - y[d], u = x[d] + y[d];
+            r2 = __builtin_fma(t, t, r2);
+            s2 = __builtin_fma(u, u, s2);
+        }
+        return s2 / r2;
+    }
+};
+
+}  // namespace lib
+}  // namespace fmma

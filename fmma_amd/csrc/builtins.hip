@@ -1,0 +1,79 @@
+// builtins.hip -- launcher tables (fmma_kernel_ops) for the FMMA_KERNEL_* ids.
+#include "../../include/fmma/detail/pair_kernels.hpp"
+#include "builtin_kernels.hpp"
+#include "builtins.hpp"
+
+namespace fmma {
+namespace lib {
+
+template <typename T, int D, class K>
+struct builtin_ops {
+    static K make(void *ctx)
+    {
+        const BuiltinParams *c = static_cast<const BuiltinParams *>(ctx);
+        K k;
+        k.p0 = c->p0;
+        k.p1 = c->p1;
+        return k;
+    }
+    static int p2p(const fmma_p2p_args *a, void *ctx, void *s)
+    {
+        return detail::launch_p2p<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
+    static int exact(const fmma_exact_args *a, void *ctx, void *s)
+    {
+        return detail::launch_exact<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
+    static int m2l_op(const fmma_m2l_op_args *a, void *ctx, void *s)
+    {
+        return detail::launch_m2l_op<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
+    static int m2l_general(const fmma_m2l_general_args *a, void *ctx, void *s)
+    {
+        return detail::launch_m2l_general<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
+    static void fill(fmma_kernel_ops *o, void *ctx)
+    {
+        o->dim = D;
+        o->dtype = sizeof(T) == 8 ? FMMA_F64 : FMMA_F32;
+        o->translation_invariant = K::translation_invariant ? 1 : 0;
+        o->ctx = ctx;
+        o->p2p = &p2p;
+        o->exact = &exact;
+        o->m2l_op = &m2l_op;
+        o->m2l_general = &m2l_general;
+    }
+};
+
+template <typename T, int D>
+static int fill_kernel(int id, void *ctx, fmma_kernel_ops *o)
+{
+    switch (id) {
+    case FMMA_KERNEL_INV_R: builtin_ops<T, D, KInvR>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_INV_R_REG: builtin_ops<T, D, KInvRReg>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_INV_DIFF: builtin_ops<T, D, KInvDiff>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_LOG_R: builtin_ops<T, D, KLogR>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_GAUSS: builtin_ops<T, D, KGauss>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_YUKAWA: builtin_ops<T, D, KYukawa>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_SUM_R2: builtin_ops<T, D, KSumR2>::fill(o, ctx); return FMMA_OK;
+    case FMMA_KERNEL_SUM2_R2: builtin_ops<T, D, KSum2R2>::fill(o, ctx); return FMMA_OK;
+    }
+    return FMMA_ERR_INVALID;
+}
+
+int builtin_kernel_ops(int id, int dim, int dtype, void *ctx, fmma_kernel_ops *out)
+{
+    if (dtype == FMMA_F64) {
+        if (dim == 1) return fill_kernel<double, 1>(id, ctx, out);
+        if (dim == 2) return fill_kernel<double, 2>(id, ctx, out);
+        if (dim == 3) return fill_kernel<double, 3>(id, ctx, out);
+    } else {
+        if (dim == 1) return fill_kernel<float, 1>(id, ctx, out);
+        if (dim == 2) return fill_kernel<float, 2>(id, ctx, out);
+        if (dim == 3) return fill_kernel<float, 3>(id, ctx, out);
+    }
+    return FMMA_ERR_INVALID;
+}
+
+}  // namespace lib
+}  // namespace fmma

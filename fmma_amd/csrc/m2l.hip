@@ -1,0 +1,384 @@
+// m2l.hip -- M2L, the one dense contraction of the FMM, on MFMA.
+//
+// The reference applies, for every box b and every box m of its interaction
+// list, Wl[b] += K_{m-b} Wm[m] as a P x P matvec (src/fmm_core.cpp:419-455,
+// matvec src/math.cpp:263-294), with K_o[kl][km] = f(X_kl, Y_km) built once per
+// level and offset (src/fmm_core.cpp:388-418).
+//
+// Here a workgroup owns 64 target boxes that share a child class sigma (the
+// sigma-children of 64 consecutive parents in Morton order) and every row
+// chunk of 64 coefficients of them.  For each of the class's 6^D - 3^D
+// offsets o it gathers the 64 partner multipoles Wm[b + o] into LDS (zero
+// columns where b + o leaves the grid) and multiplies them by the operator
+// K_o with MFMA: C[kl][b] += K_o[kl][km] * Wm[b+o][km].  The accumulators stay
+// in registers across all offsets, so Wl is written exactly once; K_o is read
+// straight from global memory (L2) in fragment order.  All levels run in one
+// launch.
+//
+// Lane layouts (CDNA3/CDNA4 MFMA, wave64):
+//   16x16x4 (f64, f32): A[i][k]: lane = i + 16k;  B[k][j]: lane = j + 16k;
+//                       f32: C[i][j]: lane = j + 16*(i/4), element i%4;
+//                       f64: C[i][j]: lane = j + 16*(i%4), element i/4.
+//   16x16x32 bf16:      A[i][k]: lane = i + 16*(k/8), element k%8;
+//                       B[k][j]: lane = j + 16*(k/8), element k%8; C as above.
+#include "m2l.hpp"
+
+namespace fmma {
+namespace lib {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------- host tables
+
+void OffsetTables::build(int dim)
+{
+    D = dim;
+    far.clear();
+    cls_f.clear();
+    cls_o.clear();
+    int n7 = 1;
+    for (int d = 0; d < D; ++d) n7 *= 7;
+    std::vector<int> far_index(n7, -1);
+    for (int s = 0; s < n7; ++s) {
+        int o[3], ss = s, mx = 0;
+        for (int d = D - 1; d >= 0; --d) {
+            o[d] = ss % 7 - 3;
+            ss /= 7;
+            mx = std::max(mx, std::abs(o[d]));
+        }
+        if (mx <= 1) continue;
+        far_index[s] = (int)(far.size() / D);
+        for (int d = 0; d < D; ++d) far.push_back(o[d]);
+    }
+    n_far = (int)(far.size() / D);
+    n_cls = 0;
+    for (int c = 0; c < (1 << D); ++c) {
+        int cnt = 0;
+        for (int s = 0; s < n7; ++s) {
+            if (far_index[s] < 0) continue;
+            int o[3], ss = s;
+            for (int d = D - 1; d >= 0; --d) {
+                o[d] = ss % 7 - 3;
+                ss /= 7;
+            }
+            bool ok = true;
+            for (int d = 0; d < D; ++d) {
+                const int par = (c >> (D - 1 - d)) & 1;
+                const int lo = par ? -3 : -2, hi = par ? 2 : 3;
+                ok = ok && o[d] >= lo && o[d] <= hi;
+            }
+            if (!ok) continue;
+            cls_f.push_back(far_index[s]);
+            for (int d = 0; d < D; ++d) cls_o.push_back(o[d]);
+            ++cnt;
+        }
+        n_cls = cnt;  // 6^D - 3^D for every class
+    }
+}
+
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level)
+{
+    std::vector<M2LTask> t;
+    for (int l = leaf_level; l >= 2; --l) {
+        const unsigned npar = 1u << (D * (l - 1));
+        for (unsigned p0 = 0; p0 < npar; p0 += kTileParents)
+            for (int c = 0; c < (1 << D); ++c) {
+                const unsigned np = npar - p0 < (unsigned)kTileParents ? npar - p0 : (unsigned)kTileParents;
+                t.push_back(M2LTask{l, c, p0, (int)np});
+            }
+    }
+    return t;
+}
+
+void m2l_shape(int P, int mode, int &MP, int &KP)
+{
+    MP = (P + 15) / 16 * 16;
+    KP = mode == 2 ? (P + 31) / 32 * 32 : (P + 3) / 4 * 4;
+}
+
+size_t m2l_frag_elems(int n_far, int P, int mode)
+{
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
+    return (size_t)n_far * MP * KP;
+}
+
+// --------------------------------------------------------------- fragments
+
+template <typename T, typename CT>
+__global__ __launch_bounds__(256) void frag4_kernel(const T *Kraw, CT *Kf, int P, int MP, int KP, long total)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const long per = (long)MP * KP;
+    const long f = e / per;
+    const int r = (int)(e - f * per);
+    const int lane = r & 63, s = (r >> 6) % (KP / 4), rb = (r >> 6) / (KP / 4);
+    const int row = rb * 16 + (lane & 15), k = 4 * s + (lane >> 4);
+    Kf[e] = (row < P && k < P) ? (CT)Kraw[(f * P + row) * P + k] : CT(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void frag_bf16_kernel(const T *Kraw, __bf16 *Kh, __bf16 *Kl, int P, int MP,
+                                                        int KP, long total)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const long per = (long)MP * KP;
+    const long f = e / per;
+    const int r = (int)(e - f * per);
+    const int j = r & 7, lane = (r >> 3) & 63, s = (r >> 9) % (KP / 32), rb = (r >> 9) / (KP / 32);
+    const int row = rb * 16 + (lane & 15), k = 32 * s + 8 * (lane >> 4) + j;
+    const float v = (row < P && k < P) ? (float)Kraw[(f * P + row) * P + k] : 0.0f;
+    const __bf16 h = (__bf16)v;
+    Kh[e] = h;
+    Kl[e] = (__bf16)(v - (float)h);
+}
+
+template <typename T>
+int m2l_fragments(const T *Kraw, int n_far, int P, int mode, void *Kf, void *Kf_lo, hipStream_t st)
+{
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
+    const long total = (long)n_far * MP * KP;
+    const unsigned nb = (unsigned)((total + 255) / 256);
+    if (mode == 0)
+        hipLaunchKernelGGL((frag4_kernel<T, double>), dim3(nb), dim3(256), 0, st, Kraw, (double *)Kf, P, MP, KP, total);
+    else if (mode == 1)
+        hipLaunchKernelGGL((frag4_kernel<T, float>), dim3(nb), dim3(256), 0, st, Kraw, (float *)Kf, P, MP, KP, total);
+    else
+        hipLaunchKernelGGL((frag_bf16_kernel<T>), dim3(nb), dim3(256), 0, st, Kraw, (__bf16 *)Kf, (__bf16 *)Kf_lo, P,
+                           MP, KP, total);
+    FMMA_LAUNCHED("m2l_fragments");
+    return FMMA_OK;
+}
+
+// ---------------------------------------------------------------- contraction
+
+// Target column c of the task: sigma-child of parent p0 + c.  Writes the
+// partner box index of offset i (or -1) for the 64 columns.
+template <int D>
+__device__ __forceinline__ void partner_indices(const M2LKArgs &a, int cls, int i, int N, const int (&tc)[D],
+                                                bool tvalid, int *pidx, int tid)
+{
+    if (tid < 64) {
+        int m[D];
+        bool ok = tvalid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m[d] = tc[d] + a.cls_o[(cls * a.n_cls + i) * D + d];
+            ok = ok && m[d] >= 0 && m[d] < N;
+        }
+        pidx[tid] = ok ? (int)morton_encode<D>(m) : -1;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void target_coords(const M2LTask &t, int tid, int (&tc)[D], bool &tvalid)
+{
+    tvalid = tid < t.np;
+    int pc[D];
+    morton_decode<D>(t.p0 + (tvalid ? tid : 0), pc);
+#pragma unroll
+    for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((t.cls >> (D - 1 - d)) & 1);
+}
+
+// MODE 0: FP64 MFMA on FP64 data.  MODE 1: FP32 MFMA, per-offset FP32 sums
+// added into TW accumulators.  TW is the storage type of Wm/Wl.
+template <typename TW, int D, int MODE>
+__global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
+{
+    using CT = typename std::conditional<MODE == 0, double, float>::type;
+    using V4 = typename std::conditional<MODE == 0, d4_t, f4_t>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    CT *Bs = reinterpret_cast<CT *>(smem);
+    __shared__ int pidx[64];
+
+    const M2LTask task = a.tasks[blockIdx.x];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rb = blockIdx.y * 4 + wave;
+    const bool active = rb * 16 < a.MP;
+    const int N = 1 << task.level, P = a.P, KP = a.KP, ldb = KP + 1;
+    int tc[D];
+    bool tvalid;
+    target_coords<D>(task, tid, tc, tvalid);
+
+    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
+    const CT *Kf = static_cast<const CT *>(a.Kf) + a.kf_off[task.level];
+    const int nks = KP / 4, nrb = a.MP / 16;
+
+    TW acc[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[c][q] = TW(0);
+
+    for (int i = 0; i < a.n_cls; ++i) {
+        const int f = a.cls_f[task.cls * a.n_cls + i];
+        partner_indices<D>(a, task.cls, i, N, tc, tvalid, pidx, tid);
+        __syncthreads();
+        for (int e = tid; e < 64 * KP; e += 256) {
+            const int col = e / KP, k = e - col * KP;
+            const int pi = pidx[col];
+            CT v = CT(0);
+            if (pi >= 0 && k < P) v = (CT)Wm[(size_t)pi * P + k];
+            Bs[col * ldb + k] = v;
+        }
+        __syncthreads();
+        if (active) {
+            const CT *A = Kf + ((size_t)f * nrb + rb) * nks * 64;
+            V4 c[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) c[cb] = V4{0, 0, 0, 0};
+            for (int s = 0; s < nks; ++s) {
+                const CT av = A[s * 64 + lane];
+                const int kk = 4 * s + (lane >> 4);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    const CT bv = Bs[(16 * cb + (lane & 15)) * ldb + kk];
+                    if constexpr (MODE == 0)
+                        c[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c[cb], 0, 0, 0);
+                    else
+                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c[cb], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[cb][q] += (TW)c[cb][q];
+        }
+    }
+    if (!active) return;
+    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int col = 16 * cb + (lane & 15);
+        if (col >= task.np) continue;
+        const size_t box = ((size_t)(task.p0 + col) << D) | (unsigned)task.cls;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
+            if (row < P) Wl[box * P + row] = acc[cb][q];
+        }
+    }
+}
+
+// Split-BF16 x3: K = Kh + Kl, W = Wh + Wl (each bf16); per offset
+// C = Kl*Wh + Kh*Wl + Kh*Wh with FP32 accumulation, then added into TW.
+template <typename TW, int D>
+__global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int pidx[64];
+    const M2LTask task = a.tasks[blockIdx.x];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rb = blockIdx.y * 4 + wave;
+    const bool active = rb * 16 < a.MP;
+    const int N = 1 << task.level, P = a.P, KP = a.KP, ldb = KP + 8;
+    __bf16 *Bh = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *Bl = Bh + 64 * ldb;
+    int tc[D];
+    bool tvalid;
+    target_coords<D>(task, tid, tc, tvalid);
+
+    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
+    const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + a.kf_off[task.level]);
+    const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
+    const int nks = KP / 32, nrb = a.MP / 16;
+
+    TW acc[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[c][q] = TW(0);
+
+    for (int i = 0; i < a.n_cls; ++i) {
+        const int f = a.cls_f[task.cls * a.n_cls + i];
+        partner_indices<D>(a, task.cls, i, N, tc, tvalid, pidx, tid);
+        __syncthreads();
+        for (int e = tid; e < 64 * KP; e += 256) {
+            const int col = e / KP, k = e - col * KP;
+            const int pi = pidx[col];
+            float v = 0.0f;
+            if (pi >= 0 && k < P) v = (float)Wm[(size_t)pi * P + k];
+            const __bf16 h = (__bf16)v;
+            Bh[col * ldb + k] = h;
+            Bl[col * ldb + k] = (__bf16)(v - (float)h);
+        }
+        __syncthreads();
+        if (active) {
+            const size_t abase = ((size_t)f * nrb + rb) * nks * 64;
+            f4_t c[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) c[cb] = f4_t{0, 0, 0, 0};
+            for (int s = 0; s < nks; ++s) {
+                const bf8_t ah = Kh[abase + s * 64 + lane];
+                const bf8_t al = Kl[abase + s * 64 + lane];
+                const int kk = 32 * s + 8 * (lane >> 4);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    const int off = (16 * cb + (lane & 15)) * ldb + kk;
+                    const bf8_t bh = *reinterpret_cast<const bf8_t *>(Bh + off);
+                    const bf8_t bl = *reinterpret_cast<const bf8_t *>(Bl + off);
+                    c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c[cb], 0, 0, 0);
+                    c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c[cb], 0, 0, 0);
+                    c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c[cb], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[cb][q] += (TW)c[cb][q];
+        }
+    }
+    if (!active) return;
+    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int col = 16 * cb + (lane & 15);
+        if (col >= task.np) continue;
+        const size_t box = ((size_t)(task.p0 + col) << D) | (unsigned)task.cls;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = 16 * rb + 4 * (lane >> 4) + q;
+            if (row < P) Wl[box * P + row] = acc[cb][q];
+        }
+    }
+}
+
+template <typename T, int D>
+int m2l_apply(const M2LKArgs &a, int n_tasks, int mode, hipStream_t st)
+{
+    if (n_tasks <= 0) return FMMA_OK;
+    const dim3 grid((unsigned)n_tasks, (unsigned)((a.MP + 63) / 64));
+    if (mode == 0) {
+        if constexpr (std::is_same<T, double>::value) {
+            const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
+            hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
+        } else {
+            return FMMA_ERR_INVALID;
+        }
+    } else if (mode == 1) {
+        const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
+        hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
+    } else {
+        const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);
+        hipLaunchKernelGGL((m2l_bf16x3<T, D>), grid, dim3(256), lds, st, a);
+    }
+    FMMA_LAUNCHED("m2l_apply");
+    return FMMA_OK;
+}
+
+template int m2l_fragments<double>(const double *, int, int, int, void *, void *, hipStream_t);
+template int m2l_fragments<float>(const float *, int, int, int, void *, void *, hipStream_t);
+template int m2l_apply<double, 1>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<double, 2>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<double, 3>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<float, 1>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<float, 2>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<float, 3>(const M2LKArgs &, int, int, hipStream_t);
+
+}  // namespace lib
+}  // namespace fmma

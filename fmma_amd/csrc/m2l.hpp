@@ -1,0 +1,63 @@
+// m2l.hpp -- M2L as batched MFMA contractions (m2l.hip).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace fmma {
+namespace lib {
+
+constexpr int kMaxLevels = 16;
+constexpr int kTileParents = 64;  // target boxes per workgroup (one child class of 64 parents)
+
+// Separation vectors.  "far" = every o in [-3,3]^D with max|o_d| >= 2, in
+// row-major order (the reference's 7^D operator table minus the near block,
+// src/fmm_core.cpp:391-402).  A target box's child class sigma (its
+// coordinate parities) fixes which far offsets are interactions: o_d in
+// [-2,3] when b_d is even, [-3,2] when odd -- the children of the parent's
+// neighbours (src/fmm_core.cpp:21-37) -- 6^D - 3^D of them for every class.
+struct OffsetTables {
+    int D = 0, n_far = 0, n_cls = 0;
+    std::vector<int> far;    // [n_far][D]
+    std::vector<int> cls_f;  // [2^D][n_cls] index into far
+    std::vector<int> cls_o;  // [2^D][n_cls][D]
+    void build(int dim);
+};
+
+struct M2LTask {
+    int level, cls;
+    unsigned p0;
+    int np;
+};
+
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level);
+
+// Kernel arguments of the MFMA contraction.
+struct M2LKArgs {
+    const M2LTask *tasks;
+    const int *cls_f;
+    const int *cls_o;
+    int n_cls;
+    int P, MP, KP;              // MP = 16*ceil(P/16) rows, KP = padded contraction length
+    long lvl_off[kMaxLevels];   // element offset of level l in the Wm / Wl arrays
+    long kf_off[kMaxLevels];    // element offset of level l in the fragment arrays
+    const void *Wm;
+    void *Wl;
+    const void *Kf;             // operator fragments (f64 / f32), or bf16 hi part
+    const void *Kf_lo;          // bf16 lo part
+};
+
+// Shape of the operator fragments for a precision: rows MP, contraction KP.
+void m2l_shape(int P, int precision, int &MP, int &KP);
+size_t m2l_frag_elems(int n_far, int P, int precision);  // elements per level (per part for bf16)
+
+// Kraw [n_far][P][P] (TYPE) -> fragments of level l.
+template <typename T>
+int m2l_fragments(const T *Kraw, int n_far, int P, int precision, void *Kf, void *Kf_lo, hipStream_t st);
+
+// All levels at once: one workgroup per task, blockDim 256, grid.y = row chunks.
+template <typename T, int D>
+int m2l_apply(const M2LKArgs &a, int n_tasks, int precision, hipStream_t st);
+
+}  // namespace lib
+}  // namespace fmma

@@ -1,0 +1,686 @@
+// plan.hip -- the solve driver and the C-ABI of libfmma_hip.
+//
+// FMMA<TYPE,DIM>::solve / fmm / exact (src/fmma.cpp:143-207, src/fmm.cpp:15-144)
+// become a plan object that owns grow-only device buffers and runs every
+// stage as stream-ordered kernels:
+//   bounding cube -> leaf keys -> stable radix sort -> box ranges -> gather
+//   -> P2M -> M2M (levels L-1..2) -> M2L (levels 2..L, one MFMA launch)
+//   -> L2L (levels 2..L-1) -> L2P -> P2P (writes ans in caller order).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "boxops.hpp"
+#include "builtins.hpp"
+#include "m2l.hpp"
+#include "tree.hpp"
+
+namespace fmma {
+namespace lib {
+
+static thread_local std::string g_err = "";
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return FMMA_ERR_HIP;
+}
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap) return FMMA_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return hip_fail(e, "hipMalloc");
+        }
+        cap = bytes;
+        return FMMA_OK;
+    }
+    template <class X>
+    X *as() const { return static_cast<X *>(p); }
+    ~DBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// ------------------------------------------------ host Chebyshev (FP64, libm)
+// T_j(x) = cos(j acos x), S_n(x,y) = 1/n + sum_j (2/n) T_j(x) T_j(y)
+// (src/math.cpp:14-50); nodes c_k = cos((2k+1) pi / (2n)) (src/fmm_core.cpp:240-243).
+static double host_T(int j, double x) { return std::cos(j * std::acos(x)); }
+
+static double host_S(int n, double x, double y)
+{
+    double s = 1.0 / n;
+    for (int j = 1; j < n; ++j) s += 2.0 / n * host_T(j, x) * host_T(j, y);
+    return s;
+}
+
+static const char *kStageNames[] = {"tree", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P"};
+constexpr int kStages = 7;
+
+struct Plan {
+    int dim = 3, dtype = FMMA_F64;
+    std::string solver = "exact";
+    int poly_ord = 1, depth = -1, trans_sym = 0, nrn_N = -1;
+    int kernel_id = FMMA_KERNEL_INV_R;
+    BuiltinParams kparams;
+    fmma_kernel_ops ops{};
+    int precision = FMMA_M2L_NATIVE;
+    hipStream_t stream = nullptr;
+    bool timing = true;
+    std::map<std::string, double> time_log;
+    std::vector<std::string> names_out;
+    hipEvent_t ev[kStages + 1] = {};
+    bool events_ready = false;
+
+    DBuf geom, bbox_part;
+    DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
+    DBuf t_keys, t_vals, t_keys_alt, t_vals_alt, t_hist, t_pos, t_start;
+    DBuf Wm, Wl, Kraw, Kf, Kf_lo, l2p_out, exact_part;
+    DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks;
+    DBuf h_t, h_s, h_w, h_ans;
+
+    // cached set-up keyed by (poly_ord, leaf level, mode)
+    int set_n = -1, set_L = -1, set_mode = -1;
+    OffsetTables offs;
+    int n_tasks = 0;
+    long lvl_off[kMaxLevels + 1] = {};
+    long kf_off[kMaxLevels] = {};
+    int last_L = -1, last_shared = 0;
+    long last_nt = 0, last_ns = 0;
+
+    ~Plan()
+    {
+        if (events_ready)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+
+    size_t tsize() const { return dtype == FMMA_F64 ? 8 : 4; }
+    int P() const
+    {
+        int p = 1;
+        for (int d = 0; d < dim; ++d) p *= poly_ord + 1;
+        return p;
+    }
+    int mode() const
+    {
+        if (precision == FMMA_M2L_BF16X3) return 2;
+        if (precision == FMMA_M2L_FP32 || dtype == FMMA_F32) return 1;
+        return 0;
+    }
+    void mark(int i)
+    {
+        if (timing && events_ready) (void)hipEventRecord(ev[i], stream);
+    }
+};
+
+template <typename T>
+static void to_T(const std::vector<double> &src, std::vector<unsigned char> &dst)
+{
+    dst.resize(src.size() * sizeof(T));
+    T *o = reinterpret_cast<T *>(dst.data());
+    for (size_t i = 0; i < src.size(); ++i) o[i] = (T)src[i];
+}
+
+static int upload(DBuf &b, const void *src, size_t bytes, hipStream_t st)
+{
+    int rc = b.ensure(bytes);
+    if (rc) return rc;
+    if (bytes) FMMA_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return FMMA_OK;
+}
+
+// Constant tables that depend on (poly_ord, dim, dtype, leaf level, M2L mode).
+static int setup(Plan &pl, int L)
+{
+    const int n = pl.poly_ord + 1, D = pl.dim, P = pl.P(), mode = pl.mode();
+    hipStream_t st = pl.stream;
+    int rc;
+    if (pl.set_n != n || pl.offs.D != D) {
+        std::vector<double> nodes(n), coef(n * n), tab(2 * n * n);
+        for (int k = 0; k < n; ++k) nodes[k] = std::cos((2.0 * k + 1.0) / (2 * pl.poly_ord + 2) * M_PI);
+        // nodes as TYPE, as the reference stores them (chebyshev_node_all)
+        std::vector<double> tn(n);
+        for (int k = 0; k < n; ++k) tn[k] = pl.dtype == FMMA_F64 ? nodes[k] : (double)(float)nodes[k];
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j) coef[k * n + j] = j == 0 ? 1.0 / n : 2.0 / n * host_T(j, tn[k]);
+        for (int s = 0; s < 2; ++s)
+            for (int ko = 0; ko < n; ++ko)
+                for (int ki = 0; ki < n; ++ki)
+                    tab[(s * n + ko) * n + ki] = host_S(n, tn[ko], (tn[ki] + (2 * s - 1)) / 2);
+        std::vector<unsigned char> cb, tb;
+        if (pl.dtype == FMMA_F64) {
+            to_T<double>(coef, cb);
+            to_T<double>(tab, tb);
+        } else {
+            to_T<float>(coef, cb);
+            to_T<float>(tab, tb);
+        }
+        if ((rc = upload(pl.d_nodes, tn.data(), n * sizeof(double), st))) return rc;
+        if ((rc = upload(pl.d_coef, cb.data(), cb.size(), st))) return rc;
+        if ((rc = upload(pl.d_tab, tb.data(), tb.size(), st))) return rc;
+        pl.offs.build(D);
+        if ((rc = upload(pl.d_far, pl.offs.far.data(), pl.offs.far.size() * sizeof(int), st))) return rc;
+        if ((rc = upload(pl.d_cls_f, pl.offs.cls_f.data(), pl.offs.cls_f.size() * sizeof(int), st))) return rc;
+        if ((rc = upload(pl.d_cls_o, pl.offs.cls_o.data(), pl.offs.cls_o.size() * sizeof(int), st))) return rc;
+        // keep host vectors alive until the copies are done
+        FMMA_HIP(hipStreamSynchronize(st));
+        pl.set_n = n;
+        pl.set_L = -1;
+    }
+    if (pl.set_L != L || pl.set_mode != mode) {
+        long off = 0;
+        for (int l = 0; l <= L; ++l) {
+            pl.lvl_off[l] = off;
+            off += (long)P << (D * l);
+        }
+        pl.lvl_off[L + 1] = off;
+        if ((rc = pl.Wm.ensure(off * pl.tsize()))) return rc;
+        if ((rc = pl.Wl.ensure(off * pl.tsize()))) return rc;
+        if (L >= 2) {
+            const size_t fe = m2l_frag_elems(pl.offs.n_far, P, mode);
+            const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
+            for (int l = 0; l < kMaxLevels; ++l) pl.kf_off[l] = l >= 2 ? (long)(l - 2) * (long)fe : 0;
+            if ((rc = pl.Kf.ensure(fe * esz * (L - 1)))) return rc;
+            if (mode == 2 && (rc = pl.Kf_lo.ensure(fe * esz * (L - 1)))) return rc;
+            if ((rc = pl.Kraw.ensure((size_t)pl.offs.n_far * P * P * pl.tsize()))) return rc;
+            std::vector<M2LTask> tasks = make_m2l_tasks(D, L);
+            pl.n_tasks = (int)tasks.size();
+            if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
+            FMMA_HIP(hipStreamSynchronize(st));
+        } else {
+            pl.n_tasks = 0;
+        }
+        pl.set_L = L;
+        pl.set_mode = mode;
+    }
+    return FMMA_OK;
+}
+
+// Sort one point set into its leaf boxes: keys -> radix sort -> ranges -> gather.
+template <typename T, int D>
+static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool with_w, DBuf &keys, DBuf &vals,
+                      DBuf &keys_alt, DBuf &vals_alt, DBuf &hist, DBuf &pos, DBuf *wbuf, DBuf &start)
+{
+    const unsigned nbox = 1u << (D * L);
+    int rc;
+    if ((rc = keys.ensure(n * 4)) || (rc = vals.ensure(n * 4)) || (rc = keys_alt.ensure(n * 4)) ||
+        (rc = vals_alt.ensure(n * 4)) || (rc = hist.ensure(radix_hist_words(n) * 4)) ||
+        (rc = pos.ensure((size_t)n * D * sizeof(T))) || (rc = start.ensure(((size_t)nbox + 1) * 4)))
+        return rc;
+    if (with_w && (rc = wbuf->ensure((size_t)n * sizeof(T)))) return rc;
+    const Geom<T> *g = pl.geom.as<Geom<T>>();
+    if ((rc = leaf_keys<T, D>(pts, n, g, L, keys.as<unsigned>(), pl.stream))) return rc;
+    SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
+                   hist.as<unsigned>()};
+    if ((rc = radix_sort(sb, n, D * L, pl.stream))) return rc;
+    if ((rc = box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream))) return rc;
+    return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, pl.stream);
+}
+
+template <typename T, int D>
+static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+{
+    const int L = pl.depth - 1;
+    if (D * L > 30 || L >= kMaxLevels)
+        return fail(FMMA_ERR_UNSUPPORTED, "Depth " + std::to_string(pl.depth) + " too deep for dimension " +
+                                              std::to_string(D) + " (at most 2^30 leaf boxes)");
+    const int n = pl.poly_ord + 1, P = pl.P(), mode = pl.mode();
+    if (P > kMaxP) return fail(FMMA_ERR_UNSUPPORTED, "(poly_ord+1)^DIM > 512 is not supported");
+    int rc;
+    if ((rc = setup(pl, L))) return rc;
+    hipStream_t st = pl.stream;
+    const bool shared = (const void *)tg == (const void *)sr && nt == ns;
+    const unsigned nbox = 1u << (D * L);
+
+    pl.mark(0);
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+        return rc;
+    if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), L, st))) return rc;
+    if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
+                               pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start)))
+        return rc;
+    const T *t_pos = pl.s_pos.as<T>();
+    const unsigned *t_start = pl.s_start.as<unsigned>(), *t_perm = pl.s_vals.as<unsigned>();
+    if (!shared) {
+        if ((rc = build_tree<T, D>(pl, tg, nullptr, nt, L, false, pl.t_keys, pl.t_vals, pl.t_keys_alt,
+                                   pl.t_vals_alt, pl.t_hist, pl.t_pos, nullptr, pl.t_start)))
+            return rc;
+        t_pos = pl.t_pos.as<T>();
+        t_start = pl.t_start.as<unsigned>();
+        t_perm = pl.t_vals.as<unsigned>();
+    }
+    pl.mark(1);
+
+    T *Wm = pl.Wm.as<T>(), *Wl = pl.Wl.as<T>();
+    P2MArgs<T> pa{pl.s_start.as<unsigned>(), pl.s_pos.as<T>(), pl.s_w.as<T>(), ns, pl.geom.as<Geom<T>>(), L,
+                  pl.d_coef.as<T>(), n, P, Wm + pl.lvl_off[L]};
+    if ((rc = p2m<T, D>(pa, nbox, st))) return rc;
+    pl.mark(2);
+
+    for (int l = L - 1; l >= 2; --l)
+        if ((rc = m2m<T, D>(Wm + pl.lvl_off[l + 1], Wm + pl.lvl_off[l], 1L << (D * l), pl.d_tab.as<T>(), n, P, st)))
+            return rc;
+    pl.mark(3);
+
+    if (L >= 2) {
+        const bool sym = pl.trans_sym == 1 || pl.ops.translation_invariant;
+        if (sym) {
+            for (int l = 2; l <= L; ++l) {
+                fmma_m2l_op_args oa{D, pl.dtype, n, P, pl.d_nodes.as<double>(), l,
+                                    &pl.geom.as<Geom<T>>()->Len, pl.d_far.as<int>(), pl.offs.n_far,
+                                    pl.Kraw.p};
+                if ((rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st))) return rc == FMMA_ERR_HIP ? rc : rc;
+                char *kf = pl.Kf.as<char>(), *kl = pl.Kf_lo.as<char>();
+                const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
+                if ((rc = m2l_fragments<T>(pl.Kraw.as<T>(), pl.offs.n_far, P, mode, kf + pl.kf_off[l] * esz,
+                                           mode == 2 ? kl + pl.kf_off[l] * esz : nullptr, st)))
+                    return rc;
+            }
+            M2LKArgs ka{};
+            ka.tasks = pl.d_tasks.as<M2LTask>();
+            ka.cls_f = pl.d_cls_f.as<int>();
+            ka.cls_o = pl.d_cls_o.as<int>();
+            ka.n_cls = pl.offs.n_cls;
+            ka.P = P;
+            m2l_shape(P, mode, ka.MP, ka.KP);
+            for (int l = 0; l < kMaxLevels; ++l) {
+                ka.lvl_off[l] = l <= L ? pl.lvl_off[l] : 0;
+                ka.kf_off[l] = pl.kf_off[l];
+            }
+            ka.Wm = Wm;
+            ka.Wl = Wl;
+            ka.Kf = pl.Kf.p;
+            ka.Kf_lo = pl.Kf_lo.p;
+            if ((rc = m2l_apply<T, D>(ka, pl.n_tasks, mode, st))) return rc;
+        } else {
+            for (int l = 2; l <= L; ++l) {
+                fmma_m2l_general_args ga{D, pl.dtype, n, P, pl.d_nodes.as<double>(), l,
+                                         &pl.geom.as<Geom<T>>()->Len, pl.geom.as<Geom<T>>()->origin,
+                                         Wm + pl.lvl_off[l], Wl + pl.lvl_off[l]};
+                if ((rc = pl.ops.m2l_general(&ga, pl.ops.ctx, st))) return rc;
+            }
+        }
+    }
+    pl.mark(4);
+
+    for (int l = 2; l + 1 <= L; ++l)
+        if ((rc = l2l<T, D>(Wl + pl.lvl_off[l], Wl + pl.lvl_off[l + 1], 1L << (D * l), pl.d_tab.as<T>(), n, P, st)))
+            return rc;
+    pl.mark(5);
+
+    const T *base = nullptr;
+    if (L >= 2) {
+        if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
+        L2PArgs<T> la{t_start, t_pos, nt, pl.geom.as<Geom<T>>(), L, pl.d_coef.as<T>(), n, P, Wl + pl.lvl_off[L],
+                      pl.l2p_out.as<T>()};
+        if ((rc = l2p<T, D>(la, nbox, st))) return rc;
+        base = pl.l2p_out.as<T>();
+    }
+    pl.mark(6);
+
+    fmma_p2p_args qa{D, pl.dtype, (long)nbox, L, t_start, pl.s_start.as<unsigned>(), t_pos, pl.s_pos.as<T>(),
+                     pl.s_w.as<T>(), nt, ns, base, t_perm, ans, L < 2 ? 1 : 0};
+    if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+    pl.mark(7);
+    pl.last_L = L;
+    pl.last_shared = shared ? 1 : 0;
+    pl.last_nt = nt;
+    pl.last_ns = ns;
+    return FMMA_OK;
+}
+
+template <typename T, int D>
+static int run_exact(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+{
+    long waves = (nt + 63) / 64;
+    int split = 1;
+    if (waves < 8192) {
+        long s = 8192 / (waves > 0 ? waves : 1);
+        long smax = ns / 1024;
+        s = s < smax ? s : smax;
+        s = s < 256 ? s : 256;
+        split = s > 1 ? (int)s : 1;
+    }
+    int rc;
+    if (split > 1 && (rc = pl.exact_part.ensure((size_t)split * nt * sizeof(T)))) return rc;
+    fmma_exact_args ea{D, pl.dtype, nt, ns, tg, w, sr, ans, split > 1 ? pl.exact_part.p : nullptr, split};
+    return pl.ops.exact(&ea, pl.ops.ctx, pl.stream);
+}
+
+template <typename T, int D>
+static int solve_typed(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans,
+                       bool is_fmm)
+{
+    if (is_fmm)
+        return run_fmm<T, D>(pl, nt, (const T *)tg, (const T *)w, ns, (const T *)sr, (T *)ans);
+    return run_exact<T, D>(pl, nt, (const T *)tg, (const T *)w, ns, (const T *)sr, (T *)ans);
+}
+
+static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans)
+{
+    const std::string &s = pl.solver;
+    const bool is_fmm = s == "fmm";
+    if (!is_fmm && s != "exact" && s != "exact_matvec") {
+        if (s == "tree" || s == "nrnmm")
+            return fail(FMMA_ERR_UNSUPPORTED, "solver type " + s + " does not run on the GPU");
+        return fail(FMMA_ERR_SOLVER, "solve type " + s + " not undefined");
+    }
+    if (nt < 0 || ns < 0) return fail(FMMA_ERR_INVALID, "negative size");
+    if (is_fmm && pl.depth <= 0) {
+        // auto Depth from the target count, kept afterwards (src/fmm.cpp:23-28)
+        pl.depth = (int)(std::log((double)nt) / pl.dim) + 1;
+        if (pl.depth <= 0) pl.depth = 1;
+    }
+    if (nt == 0 || ns == 0) return FMMA_OK;
+    if (!tg || !sr || !ans) return fail(FMMA_ERR_INVALID, "null pointer");
+    if (!pl.ops.p2p || pl.ops.dim != pl.dim || pl.ops.dtype != pl.dtype)
+        return fail(FMMA_ERR_INVALID, "kernel launchers do not match the plan's DIM/TYPE");
+    if (pl.timing && !pl.events_ready) {
+        for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
+        pl.events_ready = true;
+    }
+    if (!is_fmm) pl.mark(0);
+    int rc;
+    if (pl.dtype == FMMA_F64) {
+        if (pl.dim == 1) rc = solve_typed<double, 1>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        else if (pl.dim == 2) rc = solve_typed<double, 2>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        else rc = solve_typed<double, 3>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+    } else {
+        if (pl.dim == 1) rc = solve_typed<float, 1>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        else if (pl.dim == 2) rc = solve_typed<float, 2>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        else rc = solve_typed<float, 3>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+    }
+    if (rc) return rc;
+    if (!is_fmm) pl.mark(7);
+    FMMA_HIP(hipStreamSynchronize(pl.stream));
+    if (pl.timing && pl.events_ready) {
+        float total = 0.f;
+        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[7]));
+        if (is_fmm) {
+            for (int i = 0; i < kStages; ++i) {
+                float ms = 0.f;
+                FMMA_HIP(hipEventElapsedTime(&ms, pl.ev[i], pl.ev[i + 1]));
+                pl.time_log[kStageNames[i]] += ms;
+            }
+        }
+        pl.time_log["total time(" + s + ")"] = total;
+    }
+    return FMMA_OK;
+}
+
+}  // namespace lib
+}  // namespace fmma
+
+using fmma::lib::Plan;
+
+struct fmma_plan_s {
+    Plan pl;
+};
+
+#define FMMA_GUARD(...)                                                          \
+    try {                                                                        \
+        __VA_ARGS__;                                                             \
+    } catch (const std::bad_alloc &) {                                           \
+        return fmma::lib::fail_oom();                                            \
+    } catch (...) {                                                              \
+        fmma::lib::set_error("unexpected C++ exception");                         \
+        return FMMA_ERR_INVALID;                                                 \
+    }
+
+namespace fmma {
+namespace lib {
+int fail_oom() { return fail(FMMA_ERR_INVALID, "host allocation failed"); }
+}  // namespace lib
+}  // namespace fmma
+
+extern "C" {
+
+int fmma_plan_create(fmma_plan *plan, int dim, int dtype)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan pointer");
+    *plan = nullptr;
+    if (dim < 1 || dim > 3) return fmma::lib::fail(FMMA_ERR_INVALID, "DIM must be 1, 2 or 3");
+    if (dtype != FMMA_F64 && dtype != FMMA_F32) return fmma::lib::fail(FMMA_ERR_INVALID, "dtype must be F64 or F32");
+    FMMA_GUARD({
+        fmma_plan p = new fmma_plan_s();
+        p->pl.dim = dim;
+        p->pl.dtype = dtype;
+        fmma::lib::builtin_kernel_ops(FMMA_KERNEL_INV_R, dim, dtype, &p->pl.kparams, &p->pl.ops);
+        *plan = p;
+    });
+    return FMMA_OK;
+}
+
+int fmma_plan_destroy(fmma_plan plan)
+{
+    delete plan;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_solver_type(fmma_plan plan, const char *type)
+{
+    if (!plan || !type) return fmma::lib::fail(FMMA_ERR_INVALID, "null argument");
+    FMMA_GUARD(plan->pl.solver = type);
+    return FMMA_OK;
+}
+
+int fmma_plan_set_poly_ord(fmma_plan plan, int poly_ord)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    if (poly_ord < 0 || poly_ord + 1 > fmma::lib::kMaxNodes)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "poly_ord must be in [0, 15]");
+    plan->pl.poly_ord = poly_ord;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_depth(fmma_plan plan, int depth)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.depth = depth;
+    return FMMA_OK;
+}
+
+int fmma_plan_get_depth(fmma_plan plan, int *depth)
+{
+    if (!plan || !depth) return fmma::lib::fail(FMMA_ERR_INVALID, "null argument");
+    *depth = plan->pl.depth;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_trans_sym_flag(fmma_plan plan, int flag)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.trans_sym = flag;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_nrn_N(fmma_plan plan, int nrn_N)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.nrn_N = nrn_N;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_kernel(fmma_plan plan, int kernel_id, const double *params, int nparams)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    if (nparams > 0 && !params) return fmma::lib::fail(FMMA_ERR_INVALID, "null params");
+    fmma::lib::BuiltinParams bp;
+    if (nparams > 0) bp.p0 = params[0];
+    if (kernel_id == FMMA_KERNEL_GAUSS) {
+        if (!(nparams > 0 && bp.p0 > 0.0)) return fmma::lib::fail(FMMA_ERR_INVALID, "gauss needs h > 0");
+        bp.p1 = 1.0 / (2.0 * bp.p0 * bp.p0);
+    }
+    if (kernel_id == FMMA_KERNEL_YUKAWA && nparams < 1)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "yukawa needs kappa");
+    fmma_kernel_ops o{};
+    plan->pl.kparams = bp;
+    const int rc = fmma::lib::builtin_kernel_ops(kernel_id, plan->pl.dim, plan->pl.dtype, &plan->pl.kparams, &o);
+    if (rc) return fmma::lib::fail(rc, "unknown kernel id " + std::to_string(kernel_id));
+    plan->pl.ops = o;
+    plan->pl.kernel_id = kernel_id;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_kernel_ops(fmma_plan plan, const fmma_kernel_ops *ops)
+{
+    if (!plan || !ops) return fmma::lib::fail(FMMA_ERR_INVALID, "null argument");
+    if (ops->dim != plan->pl.dim || ops->dtype != plan->pl.dtype || !ops->p2p || !ops->exact || !ops->m2l_op ||
+        !ops->m2l_general)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "kernel table incomplete or of another DIM/TYPE");
+    plan->pl.ops = *ops;
+    plan->pl.kernel_id = FMMA_KERNEL_CUSTOM;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_m2l_precision(fmma_plan plan, int precision)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    if (precision < FMMA_M2L_NATIVE || precision > FMMA_M2L_BF16X3)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "unknown M2L precision");
+    plan->pl.precision = precision;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_stream(fmma_plan plan, void *stream)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.stream = (hipStream_t)stream;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_timing(fmma_plan plan, int enable)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.timing = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                    const void *sources, void *ans)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    FMMA_GUARD(return fmma::lib::solve_device(plan->pl, n_t, targets, weights, n_s, sources, ans));
+}
+
+int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                         const void *sources, void *ans)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    Plan &pl = plan->pl;
+    if (n_t < 0 || n_s < 0) return fmma::lib::fail(FMMA_ERR_INVALID, "negative size");
+    const size_t ts = pl.tsize(), D = pl.dim;
+    if (n_t == 0 || n_s == 0) return fmma::lib::solve_device(pl, n_t, targets, weights, n_s, sources, ans);
+    if (!targets || !sources || !ans) return fmma::lib::fail(FMMA_ERR_INVALID, "null pointer");
+    int rc;
+    hipStream_t st = pl.stream;
+    const bool shared = targets == sources && n_t == n_s;
+    if ((rc = fmma::lib::upload(pl.h_s, sources, n_s * D * ts, st))) return rc;
+    if (!shared && (rc = fmma::lib::upload(pl.h_t, targets, n_t * D * ts, st))) return rc;
+    if (weights && (rc = fmma::lib::upload(pl.h_w, weights, n_s * ts, st))) return rc;
+    if ((rc = fmma::lib::upload(pl.h_ans, ans, n_t * ts, st))) return rc;
+    const void *dt = shared ? pl.h_s.p : pl.h_t.p;
+    FMMA_GUARD(rc = fmma::lib::solve_device(pl, n_t, dt, weights ? pl.h_w.p : nullptr, n_s, pl.h_s.p, pl.h_ans.p));
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(ans, pl.h_ans.p, n_t * ts, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fmma::lib::hip_fail(e, "copy ans to host");
+    return FMMA_OK;
+}
+
+int fmma_plan_time_log(fmma_plan plan, const char **names, double *ms, int max)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    Plan &pl = plan->pl;
+    pl.names_out.clear();
+    int i = 0;
+    for (const auto &kv : pl.time_log) {
+        pl.names_out.push_back(kv.first);
+        ++i;
+    }
+    const int n = i;
+    i = 0;
+    for (const auto &kv : pl.time_log) {
+        if (i >= max) break;
+        if (names) names[i] = pl.names_out[i].c_str();
+        if (ms) ms[i] = kv.second;
+        ++i;
+    }
+    return n;
+}
+
+int fmma_plan_clear_time_log(fmma_plan plan)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.time_log.clear();
+    return FMMA_OK;
+}
+
+int fmma_plan_level_coeffs(fmma_plan plan, int which, int level, const void **ptr, long *n_boxes, int *P)
+{
+    if (!plan || !ptr || !n_boxes || !P) return fmma::lib::fail(FMMA_ERR_INVALID, "null argument");
+    Plan &pl = plan->pl;
+    if (pl.last_L < 0 || level < 0 || level > pl.last_L)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "no such level in the last fmm solve");
+    const char *base = (which == 0 ? pl.Wm : pl.Wl).as<const char>();
+    *ptr = base + pl.lvl_off[level] * pl.tsize();
+    *n_boxes = 1L << (pl.dim * level);
+    *P = pl.P();
+    return FMMA_OK;
+}
+
+int fmma_plan_point_boxes(fmma_plan plan, int which, long n, long *boxes)
+{
+    if (!plan || !boxes) return fmma::lib::fail(FMMA_ERR_INVALID, "null argument");
+    Plan &pl = plan->pl;
+    if (pl.last_L < 0) return fmma::lib::fail(FMMA_ERR_INVALID, "no fmm solve yet");
+    const bool tgt = which == 1 && pl.last_shared == 0;
+    const long cnt = which == 1 ? pl.last_nt : pl.last_ns;
+    if (n != cnt) return fmma::lib::fail(FMMA_ERR_INVALID, "n does not match the last solve");
+    FMMA_GUARD({
+        std::vector<unsigned> keys(n), perm(n);
+        const fmma::lib::DBuf &kb = tgt ? pl.t_keys : pl.s_keys, &vb = tgt ? pl.t_vals : pl.s_vals;
+        FMMA_HIP(hipStreamSynchronize(pl.stream));
+        FMMA_HIP(hipMemcpy(keys.data(), kb.p, n * 4, hipMemcpyDeviceToHost));
+        FMMA_HIP(hipMemcpy(perm.data(), vb.p, n * 4, hipMemcpyDeviceToHost));
+        for (long i = 0; i < n; ++i) boxes[perm[i]] = fmma_morton_to_rowmajor(pl.dim, pl.last_L, keys[i]);
+    });
+    return FMMA_OK;
+}
+
+long fmma_morton_to_rowmajor(int dim, int level, long morton)
+{
+    int b[3] = {0, 0, 0};
+    if (dim == 1) fmma::detail::morton_decode<1>((unsigned)morton, b);
+    else if (dim == 2) fmma::detail::morton_decode<2>((unsigned)morton, b);
+    else fmma::detail::morton_decode<3>((unsigned)morton, b);
+    long id = 0;
+    for (int d = 0; d < dim; ++d) id = (id << level) + b[d];
+    return id;
+}
+
+const char *fmma_last_error(void) { return fmma::lib::g_err.c_str(); }
+
+const char *fmma_version(void) { return "fmma-mi355x 0.1.0"; }
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// tree.hip -- bounding cube, leaf keys, stable radix sort, box ranges.
+//
+// Replaces the reference's serial binning (src/fmm_core.cpp:255-267, which
+// pushes every source index into a std::vector per box) and its
+// get_minmax/get_origin_and_length (src/fmm_core.cpp:127-173).  Points are
+// sorted by Morton leaf key with a stable LSD radix sort, so inside a box
+// they keep ascending original index, the order of the reference's lists.
+// Box membership is bit-identical to the reference: the same IEEE
+// subtraction, division and truncation (leaf_coord in common.hpp).
+#include "tree.hpp"
+
+namespace fmma {
+namespace lib {
+
+// ------------------------------------------------------------ bounding cube
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T *b, long nb, T *part)
+{
+    T lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lo[d] = T(INFINITY);
+        hi[d] = T(-INFINITY);
+    }
+    const long stride = (long)gridDim.x * 256;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < na; i += stride)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T v = a[i * D + d];
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = hi[d] < v ? v : hi[d];
+        }
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nb; i += stride)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T v = b[i * D + d];
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = hi[d] < v ? v : hi[d];
+        }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        for (int o = 32; o > 0; o >>= 1) {
+            const T l2 = __shfl_xor(lo[d], o), h2 = __shfl_xor(hi[d], o);
+            lo[d] = l2 < lo[d] ? l2 : lo[d];
+            hi[d] = hi[d] < h2 ? h2 : hi[d];
+        }
+    __shared__ T s[4][2 * D];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            s[wave][d] = lo[d];
+            s[wave][D + d] = hi[d];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                s[0][d] = s[w][d] < s[0][d] ? s[w][d] : s[0][d];
+                s[0][D + d] = s[0][D + d] < s[w][D + d] ? s[w][D + d] : s[0][D + d];
+            }
+#pragma unroll
+        for (int d = 0; d < 2 * D; ++d) part[blockIdx.x * 2 * D + d] = s[0][d];
+    }
+}
+
+// Len = max_d (max_d - min_d), origin_d = (max_d + min_d)/2 - Len/2, in TYPE
+// (src/fmm_core.cpp:164-171); len_leaf = Len / (TYPE)N_L.
+template <typename T, int D>
+__global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> *g, int leaf_level)
+{
+    T lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lo[d] = T(INFINITY);
+        hi[d] = T(-INFINITY);
+    }
+    for (int p = threadIdx.x; p < np; p += 64)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T l = part[p * 2 * D + d], h = part[p * 2 * D + D + d];
+            lo[d] = l < lo[d] ? l : lo[d];
+            hi[d] = hi[d] < h ? h : hi[d];
+        }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        for (int o = 32; o > 0; o >>= 1) {
+            const T l2 = __shfl_xor(lo[d], o), h2 = __shfl_xor(hi[d], o);
+            lo[d] = l2 < lo[d] ? l2 : lo[d];
+            hi[d] = hi[d] < h2 ? h2 : hi[d];
+        }
+    if (threadIdx.x == 0) {
+#pragma clang fp contract(off)
+        T Len = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T e = hi[d] - lo[d];
+            Len = Len < e ? e : Len;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g->origin[d] = (hi[d] + lo[d]) / 2 - Len / 2;
+        for (int d = D; d < 3; ++d) g->origin[d] = T(0);
+        g->Len = Len;
+        g->len_leaf = Len / (T)(1u << leaf_level);
+    }
+}
+
+template <typename T, int D>
+int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int leaf_level, hipStream_t st)
+{
+    const long n = nt > ns ? nt : ns;
+    int nb = (int)((n + 255) / 256);
+    nb = nb < kBBoxParts ? nb : kBBoxParts;
+    nb = nb < 1 ? 1 : nb;
+    hipLaunchKernelGGL((bbox_partial<T, D>), dim3(nb), dim3(256), 0, st, t, nt, s, ns, part);
+    hipLaunchKernelGGL((bbox_final<T, D>), dim3(1), dim3(64), 0, st, part, nb, g, leaf_level);
+    FMMA_LAUNCHED("bbox");
+    return FMMA_OK;
+}
+
+// ---------------------------------------------------------------- leaf keys
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
+                                                   unsigned *keys)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int NL = 1 << leaf_level;
+    const T len = g->len_leaf;
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = leaf_coord(pts[i * D + d], g->origin[d], len, NL);
+    keys[i] = morton_encode<D>(b);
+}
+
+template <typename T, int D>
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, unsigned *keys, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    hipLaunchKernelGGL((keys_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, n, g,
+                       leaf_level, keys);
+    FMMA_LAUNCHED("leaf_keys");
+    return FMMA_OK;
+}
+
+// --------------------------------------------------------------- radix sort
+//
+// 8-bit digits, tiles of 4096 keys.  rs_hist counts digits per tile,
+// scan_exclusive turns the digit-major [256][tiles] table into scatter bases,
+// and rs_scatter ranks keys stably inside the tile: lanes of one wave that
+// share a digit are found with 8 ballots, waves are ordered through LDS
+// counts, and 16 rounds of 256 keys keep the original order across the tile.
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(256) void rs_hist(const unsigned *keys, long n, int shift, unsigned *hist,
+                                               unsigned ntiles)
+{
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long base = (long)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(long)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive prefix sum of m values in place, one workgroup of 1024 threads
+// walking the array in chunks of 4096 with a running carry.
+__global__ __launch_bounds__(1024) void scan_exclusive(unsigned *a, long m)
+{
+    __shared__ unsigned wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned carry = 0;
+    for (long base = 0; base < m; base += 4096) {
+        const long i0 = base + tid * 4;
+        unsigned v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = i0 + q < m ? a[i0 + q] : 0u;
+        const unsigned s = v[0] + v[1] + v[2] + v[3];
+        unsigned x = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        unsigned before = carry, total = 0;
+        for (int w = 0; w < 16; ++w) {
+            const unsigned ws = wsum[w];
+            if (w < wave) before += ws;
+            total += ws;
+        }
+        unsigned e = before + x - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (i0 + q < m) a[i0 + q] = e;
+            e += v[q];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void rs_scatter(const unsigned *kin, const unsigned *vin, unsigned *kout,
+                                                  unsigned *vout, long n, int shift, const unsigned *offs,
+                                                  unsigned ntiles)
+{
+    __shared__ unsigned base_s[256];
+    __shared__ unsigned run_s[256];
+    __shared__ unsigned wc[4][256];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    base_s[tid] = offs[(long)tid * ntiles + blockIdx.x];
+    run_s[tid] = 0;
+    wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
+    __syncthreads();
+    const unsigned long long below = __lanemask_lt();
+    const long tile0 = (long)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long i = tile0 + (long)r * RS_THREADS + tid;
+        const bool valid = i < n;
+        const unsigned key = valid ? kin[i] : 0u;
+        const unsigned val = valid ? (vin ? vin[i] : (unsigned)i) : 0u;
+        const unsigned dig = (key >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dig >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const unsigned rank = __popcll(peers & below);
+        if (valid && rank == 0) wc[wave][dig] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned pre = run_s[dig];
+            for (int w = 0; w < wave; ++w) pre += wc[w][dig];
+            const unsigned dst = base_s[dig] + pre + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        run_s[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+        wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void iota_kernel(unsigned *v, long n)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = (unsigned)i;
+}
+
+int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    if (bits <= 0) {
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b.vals, n);
+        FMMA_LAUNCHED("iota");
+        return FMMA_OK;
+    }
+    const unsigned ntiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
+    const int passes = (bits + 7) / 8;
+    unsigned *kin = b.keys, *kout = b.keys_alt, *vin = nullptr, *vout = b.vals_alt;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
+        hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, st, b.hist, (long)ntiles * 256);
+        hipLaunchKernelGGL(rs_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, n, shift, b.hist,
+                           ntiles);
+        FMMA_LAUNCHED("radix_sort");
+        // ping-pong; after the pass the sorted data is in (kout, vout)
+        unsigned *kprev = kin;
+        kin = kout;
+        vin = vout;
+        kout = kprev;
+        vout = (vout == b.vals_alt) ? b.vals : b.vals_alt;
+    }
+    // make (b.keys, b.vals) the sorted result
+    if (kin != b.keys) {
+        FMMA_HIP(hipMemcpyAsync(b.keys, kin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+    }
+    if (vin != b.vals) {
+        FMMA_HIP(hipMemcpyAsync(b.vals, vin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+    }
+    return FMMA_OK;
+}
+
+size_t radix_hist_words(long n)
+{
+    const long ntiles = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)(ntiles > 0 ? ntiles : 1) * 256;
+}
+
+// --------------------------------------------------------------- box ranges
+
+// start[k] = first sorted position whose key is >= k, for k in [0, nbox].
+__global__ __launch_bounds__(256) void ranges_kernel(const unsigned *keys, long n, unsigned nbox, unsigned *start)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    const long kprev = i == 0 ? -1 : (long)keys[i - 1];
+    const long kcur = i == n ? (long)nbox : (long)keys[i];
+    for (long k = kprev + 1; k <= kcur; ++k) start[k] = (unsigned)i;
+}
+
+int box_ranges(const unsigned *keys, long n, unsigned nbox, unsigned *start, hipStream_t st)
+{
+    hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, st, keys, n, nbox,
+                       start);
+    FMMA_LAUNCHED("box_ranges");
+    return FMMA_OK;
+}
+
+// ------------------------------------------------------------------ gather
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void gather_kernel(const T *pts, const T *w, const unsigned *perm, long n, T *pos,
+                                                     T *ws)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned p = perm[i];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pos[d * n + i] = pts[(long)p * D + d];
+    if (ws) ws[i] = w ? w[p] : T(1);
+}
+
+template <typename T, int D>
+int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    hipLaunchKernelGGL((gather_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, w, perm,
+                       n, pos, ws);
+    FMMA_LAUNCHED("gather");
+    return FMMA_OK;
+}
+
+#define FMMA_TREE_INST(T, D)                                                                                    \
+    template int bbox<T, D>(const T *, long, const T *, long, T *, Geom<T> *, int, hipStream_t);                \
+    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, unsigned *, hipStream_t);               \
+    template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);
+FMMA_TREE_INST(double, 1)
+FMMA_TREE_INST(double, 2)
+FMMA_TREE_INST(double, 3)
+FMMA_TREE_INST(float, 1)
+FMMA_TREE_INST(float, 2)
+FMMA_TREE_INST(float, 3)
+
+}  // namespace lib
+}  // namespace fmma

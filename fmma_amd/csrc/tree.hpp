@@ -1,0 +1,35 @@
+// tree.hpp -- tree-build entry points (tree.hip).
+#pragma once
+#include "common.hpp"
+
+namespace fmma {
+namespace lib {
+
+constexpr int kBBoxParts = 1024;  // partial min/max blocks of the bounding-cube reduction
+
+struct SortBuffers {
+    unsigned *keys;      // [n] in: leaf keys; out: sorted keys
+    unsigned *vals;      // [n] out: original index of each sorted point
+    unsigned *keys_alt;  // [n] ping-pong
+    unsigned *vals_alt;  // [n]
+    unsigned *hist;      // [radix_hist_words(n)]
+};
+
+template <typename T, int D>
+int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int leaf_level, hipStream_t st);
+
+template <typename T, int D>
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, unsigned *keys, hipStream_t st);
+
+// Stable sort of b.keys (low `bits` bits significant); b.vals gets the
+// permutation.  bits == 0 only fills b.vals with 0..n-1.
+int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st);
+size_t radix_hist_words(long n);
+
+int box_ranges(const unsigned *keys, long n, unsigned nbox, unsigned *start, hipStream_t st);
+
+template <typename T, int D>
+int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st);
+
+}  // namespace lib
+}  // namespace fmma

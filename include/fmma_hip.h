@@ -1,0 +1,229 @@
+/*
+ * fmma_hip.h -- C-ABI of the MI355X black-box FMM library (libfmma_hip.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  Each
+ * entry point names the reference (fockl/FMMA) interface it replaces.  The
+ * reference has no C-ABI of its own; its surface is the C++ class
+ * fmma::FMMA<TYPE,DIM> (include/fmma/fmma.hpp:52-170) and the pybind11 module
+ * pyfmma (cpython/pyfmma.cpp:140-168).  include/fmma/fmma.hpp in this tree
+ * rebuilds that class on top of these functions.
+ *
+ * Threading: a plan is not re-entrant (as the reference object is not:
+ * src/fmm.cpp:23-28 mutates Depth); use one plan per thread.  All work is
+ * stream-ordered on the plan's stream; the solve entry points return after
+ * that stream has finished the solve.
+ */
+#ifndef FMMA_HIP_H
+#define FMMA_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- status codes */
+enum {
+    FMMA_OK = 0,
+    FMMA_ERR_INVALID = 1,     /* bad argument (dimension, order, null pointer, ...) */
+    FMMA_ERR_SIZE = 2,        /* source_weight size != source size (src/fmm.cpp:32-35) */
+    FMMA_ERR_SOLVER = 3,      /* unknown solver type (src/fmma.cpp:183-186) */
+    FMMA_ERR_HIP = 4,         /* a HIP runtime call failed; fmma_last_error() has the text */
+    FMMA_ERR_UNSUPPORTED = 5  /* valid for the reference, not provided on the GPU */
+};
+
+/* ------------------------------------------------------------- data types */
+enum { FMMA_F64 = 0, FMMA_F32 = 1 };
+
+/* Built-in kernels f(x,y).  The reference takes any std::function
+ * (include/fmma/fmma.hpp:55-87); a C caller names one of these instead, and a
+ * C++ caller compiled with hipcc can pass its own functor (fmma_kernel_ops). */
+enum {
+    FMMA_KERNEL_INV_R = 0,     /* 1/|x-y|, inf at r=0: the reference default (fmma.hpp:55-63) */
+    FMMA_KERNEL_INV_R_REG = 1, /* 1/|x-y|, 0 at r=0 */
+    FMMA_KERNEL_INV_DIFF = 2,  /* 1/(x_0-y_0) */
+    FMMA_KERNEL_LOG_R = 3,     /* log|x-y|, 0 at r=0 */
+    FMMA_KERNEL_GAUSS = 4,     /* exp(-|x-y|^2/(2h^2)), params[0]=h */
+    FMMA_KERNEL_YUKAWA = 5,    /* exp(-kappa|x-y|)/|x-y|, 0 at r=0, params[0]=kappa */
+    FMMA_KERNEL_SUM_R2 = 6,    /* sum_d (x_d-y_d)/|x-y|^2 (test/src/fmma/fmm.cpp:82-91) */
+    FMMA_KERNEL_SUM2_R2 = 7,   /* sum_d (x_d+y_d)^2/|x-y|^2, not translation invariant
+                                  (test/src/fmma/fmm.cpp:151-161) */
+    FMMA_KERNEL_CUSTOM = 100   /* set by fmma_plan_set_kernel_ops */
+};
+
+/* Arithmetic of the M2L contraction.  The reference computes it in TYPE
+ * (src/fmm_core.cpp:451 -> src/math.cpp:263-294); the other forms are new. */
+enum {
+    FMMA_M2L_NATIVE = 0, /* TYPE: FP64 MFMA for double, FP32 MFMA for float */
+    FMMA_M2L_FP32 = 1,   /* FP32 MFMA per offset, summed over offsets in TYPE */
+    FMMA_M2L_BF16X3 = 2  /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 per offset */
+};
+
+typedef struct fmma_plan_s *fmma_plan;
+
+/* ---------------------------------------------- kernel-dependent launchers
+ * The stages whose arithmetic depends on f: P2P, the exact solver and the
+ * M2L operator tables.  The library fills this table for the built-in
+ * kernels; include/fmma/fmma_device.hpp fills it for a user functor compiled
+ * in the caller's hipcc translation unit.  All pointers are device pointers;
+ * coordinates are structure-of-arrays (dimension-major). */
+typedef struct {
+    int dim, dtype;
+    long n_target_boxes;       /* leaf boxes, Morton order */
+    int leaf_level;            /* boxes per side = 1 << leaf_level */
+    const unsigned *t_start;   /* [n_boxes+1] target ranges in sorted order */
+    const unsigned *s_start;   /* [n_boxes+1] source ranges in sorted order */
+    const void *t_pos;         /* [dim][n_t] sorted target coordinates */
+    const void *s_pos;         /* [dim][n_s] sorted source coordinates */
+    const void *s_w;           /* [n_s] sorted weights */
+    long n_t, n_s;
+    const void *base;          /* [n_t] value to add to, sorted order (L2P result), or NULL */
+    const unsigned *t_perm;    /* [n_t] original index of each sorted target */
+    void *ans;                 /* [n_t] caller's output, original order */
+    int accumulate;            /* base==NULL: 1 adds onto ans (leaf level < 2), 0 overwrites */
+} fmma_p2p_args;
+
+typedef struct {
+    int dim, dtype;
+    long n_t, n_s;
+    const void *targets;       /* [n_t][dim] caller layout (array of structures) */
+    const void *weights;       /* [n_s] or NULL for all-ones */
+    const void *sources;       /* [n_s][dim] */
+    void *ans;                 /* [n_t] */
+    void *partial;             /* [split][n_t] scratch of the library, or NULL when split == 1 */
+    int split;                 /* number of source slices summed in fixed order */
+} fmma_exact_args;
+
+/* The bounding cube lives on the device (it is reduced there); the box side
+ * at a level is len = Len / (TYPE)(1 << level), as in src/fmm_core.cpp:376. */
+typedef struct {
+    int dim, dtype;
+    int n_nodes;               /* nodes per dimension n = poly_ord + 1 */
+    int P;                     /* n^dim */
+    const double *nodes;       /* [n] Chebyshev nodes (device) */
+    int level;                 /* boxes per side 1 << level */
+    const void *Len;           /* TYPE, device: side of the bounding cube */
+    const int *offsets;        /* [n_off][dim] separation vectors o (device) */
+    int n_off;
+    void *K;                   /* [n_off][P][P] K[o][kl][km] = f(len*c_kl/2, len*(o + c_km/2)), TYPE */
+} fmma_m2l_op_args;
+
+typedef struct {
+    int dim, dtype;
+    int n_nodes, P;
+    const double *nodes;
+    int level;                 /* boxes per side 1 << level */
+    const void *Len;           /* TYPE, device */
+    const void *origin;        /* [dim] TYPE, device */
+    const void *Wm;            /* [boxes][P] multipoles, Morton order, TYPE */
+    void *Wl;                  /* [boxes][P] locals, overwritten */
+} fmma_m2l_general_args;
+
+/* Each launcher gets the table's `ctx` back (the functor's state, or the
+ * built-in kernel's parameters) and the plan's hipStream_t; it returns an
+ * FMMA_* status and must not synchronise. */
+typedef struct {
+    int dim, dtype;
+    int translation_invariant; /* 1: f(x,y) = g(x-y); M2L may use per-offset operators */
+    void *ctx;
+    int (*p2p)(const fmma_p2p_args *, void *ctx, void *stream);
+    int (*exact)(const fmma_exact_args *, void *ctx, void *stream);
+    int (*m2l_op)(const fmma_m2l_op_args *, void *ctx, void *stream);
+    int (*m2l_general)(const fmma_m2l_general_args *, void *ctx, void *stream);
+} fmma_kernel_ops;
+
+/* ------------------------------------------------------------ plan life */
+
+/* Replaces constructing fmma::FMMA<TYPE,DIM> (include/fmma/fmma.hpp:130,
+ * src/fmma.cpp:14-17).  dim in {1,2,3}; dtype FMMA_F64 or FMMA_F32.  The new
+ * plan has the reference's defaults: solver "exact", poly_ord 1, Depth -1
+ * (auto), trans_sym_flag 0, kernel FMMA_KERNEL_INV_R. */
+int fmma_plan_create(fmma_plan *plan, int dim, int dtype);
+
+/* Replaces ~FMMA (src/fmma.cpp:26-29).  Frees device memory. */
+int fmma_plan_destroy(fmma_plan plan);
+
+/* FMMA::set_solver_type (include/fmma/fmma.hpp:92-95).  "exact", "exact_matvec"
+ * and "fmm" run on the GPU; "tree" and "nrnmm" are accepted here and make
+ * solve return FMMA_ERR_UNSUPPORTED; anything else makes solve return
+ * FMMA_ERR_SOLVER, the reference's exit path (src/fmma.cpp:183-186). */
+int fmma_plan_set_solver_type(fmma_plan plan, const char *type);
+
+/* FMMA::set_poly_ord (include/fmma/fmma.hpp:108-111): n = poly_ord+1 nodes per
+ * dimension.  0 <= poly_ord <= 15. */
+int fmma_plan_set_poly_ord(fmma_plan plan, int poly_ord);
+
+/* FMMA::set_Depth (include/fmma/fmma.hpp:116-119).  <= 0 selects the auto
+ * rule (int)(ln N_targets / DIM) + 1 at the next fmm solve, which then sticks
+ * (src/fmm.cpp:23-28). */
+int fmma_plan_set_depth(fmma_plan plan, int depth);
+int fmma_plan_get_depth(fmma_plan plan, int *depth);
+
+/* FMMA::set_trans_sym_flag (include/fmma/fmma.hpp:124-127). */
+int fmma_plan_set_trans_sym_flag(fmma_plan plan, int flag);
+
+/* FMMA::set_nrn_N (include/fmma/fmma.hpp:100-103); stored for the nrnmm
+ * solver, which this library does not run. */
+int fmma_plan_set_nrn_N(fmma_plan plan, int nrn_N);
+
+/* Stands in for FMMA::set_fn / set_fn1 / set_fn2 (include/fmma/fmma.hpp:65-87)
+ * with a built-in kernel.  params may be NULL when nparams == 0. */
+int fmma_plan_set_kernel(fmma_plan plan, int kernel_id, const double *params, int nparams);
+
+/* Stands in for FMMA::set_fn with a user functor: the table is copied. */
+int fmma_plan_set_kernel_ops(fmma_plan plan, const fmma_kernel_ops *ops);
+
+/* New: M2L arithmetic, FMMA_M2L_*. */
+int fmma_plan_set_m2l_precision(fmma_plan plan, int precision);
+
+/* New: the HIP stream the plan runs on (hipStream_t; NULL = null stream). */
+int fmma_plan_set_stream(fmma_plan plan, void *stream);
+
+/* New: record per-stage HIP-event times into the time log (default on). */
+int fmma_plan_set_timing(fmma_plan plan, int enable);
+
+/* ------------------------------------------------------------------ solve */
+
+/* FMMA::solve(target, source_weight, source, ans) (src/fmma.cpp:169-207),
+ * and with weights == NULL FMMA::solve(target, source, ans)
+ * (src/fmma.cpp:143-160).  DEVICE pointers: targets [n_t][dim], weights [n_s],
+ * sources [n_s][dim], ans [n_t], all of the plan's dtype.  targets may equal
+ * sources (same pointer and count): the tree is then built once.  As in the
+ * reference, ans is overwritten when the fmm leaf level is >= 2 and added to
+ * otherwise, and left untouched when n_t or n_s is 0. */
+int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *weights,
+                    long n_s, const void *sources, void *ans);
+
+/* The same with HOST pointers: the library stages them through device
+ * memory (the copies are outside the per-stage time log). */
+int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights,
+                         long n_s, const void *sources, void *ans);
+
+/* Per-stage milliseconds, accumulated over solves like the reference's
+ * time_log (include/fmma/fmma.hpp:168, src/fmm.cpp:57-141); the total is
+ * overwritten per solve (src/fmma.cpp:189).  Returns the number of entries;
+ * names[i] stay valid until the next solve.  Pass max = 0 to count. */
+int fmma_plan_time_log(fmma_plan plan, const char **names, double *ms, int max);
+int fmma_plan_clear_time_log(fmma_plan plan);
+
+/* Device pointers to a level's coefficients after an fmm solve, Morton box
+ * order ([boxes][P], the plan's dtype), for stage checks.  which: 0 = multipole
+ * Wm, 1 = local Wl.  Also returns P and the box count of that level. */
+int fmma_plan_level_coeffs(fmma_plan plan, int which, int level, const void **ptr, long *n_boxes, int *P);
+
+/* Leaf box of every point of the last fmm solve, as the reference's row-major
+ * box id (src/fmm_core.cpp:258-267), in the caller's point order, written to
+ * a HOST array of n longs.  which: 0 = sources, 1 = targets. */
+int fmma_plan_point_boxes(fmma_plan plan, int which, long n, long *boxes);
+
+/* Morton index <-> row-major box id (src/fmm_core.cpp:205-212) at a level. */
+long fmma_morton_to_rowmajor(int dim, int level, long morton);
+
+/* Thread-local text of the last error (never NULL). */
+const char *fmma_last_error(void);
+
+/* Library version string. */
+const char *fmma_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+"""GPU parity: the HIP path against the oracle (CPU restatement of the
+reference) on the same seeded inputs.
+
+Bars: box membership bit-exact; FP64 fmm within 1e-12 relative (L2 over all
+targets) of the reference algorithm at the same order and depth; FP64 exact
+within 1e-13; FP32 results within 3e-5 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from fmma_amd import synth
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+KID = {"inv_r": 0, "inv_r_reg": 1, "inv_diff": 2, "log_r": 3, "gauss": 4, "yukawa": 5, "sum_r2": 6, "sum2_r2": 7}
+
+
+def gpu_solve(fm, solver, t, w, s, poly_ord=1, depth=-1, trans_sym=0, kernel="inv_r", params=(),
+              dtype=np.float64, precision="native", shared=False, ans0=None, plan=None):
+    import torch
+    dim = t.shape[1]
+    pl = plan or fm.Plan(dim, dtype)
+    pl.set_solver_type(solver)
+    pl.set_poly_ord(poly_ord)
+    if plan is None:
+        pl.set_Depth(depth)
+    pl.set_trans_sym_flag(trans_sym)
+    pl.set_kernel(kernel, *params)
+    pl.set_m2l_precision(precision)
+    tt = torch.from_numpy(np.ascontiguousarray(t, dtype=dtype)).cuda()
+    st = tt if shared else torch.from_numpy(np.ascontiguousarray(s, dtype=dtype)).cuda()
+    wt = None if w is None else torch.from_numpy(np.ascontiguousarray(w, dtype=dtype)).cuda()
+    a = torch.zeros(t.shape[0], dtype=tt.dtype, device="cuda") if ans0 is None else \
+        torch.from_numpy(np.ascontiguousarray(ans0, dtype=dtype)).cuda()
+    pl.solve(tt, wt, st, a)
+    torch.cuda.synchronize()
+    return a.cpu().numpy().astype(np.float64), pl
+
+
+def _inputs(case):
+    dim, nt, ns, shared, wkind, src = case["dim"], case["nt"], case["ns"], case["shared"], case["w"], case.get("src")
+    t = synth.points(11, nt, dim)
+    if shared:
+        s = t
+    elif src == "mixture":
+        s = synth.gaussian_mixture(3, ns, dim)
+    else:
+        s = synth.points(12, ns, dim)
+    n = s.shape[0]
+    w = {"u01": synth.uniform(13, n), "pm1": 2 * synth.uniform(13, n) - 1, "normal": synth.normal(13, n),
+         "ones": None}[wkind]
+    return t, w, s
+
+
+FMM_CASES = [
+    dict(id="c1_1d_invdiff", dim=1, nt=4096, ns=4096, shared=False, w="pm1", p=7, depth=-1, ts=0, k="inv_diff"),
+    dict(id="c1_1d_invdiff_ts1", dim=1, nt=4096, ns=4096, shared=False, w="pm1", p=7, depth=-1, ts=1, k="inv_diff"),
+    dict(id="2d_log", dim=2, nt=16384, ns=16384, shared=True, w="normal", p=5, depth=6, ts=1, k="log_r"),
+    dict(id="2d_log_ts0", dim=2, nt=3000, ns=3000, shared=True, w="normal", p=5, depth=5, ts=0, k="log_r"),
+    dict(id="3d_invr_d4", dim=3, nt=32768, ns=32768, shared=True, w="u01", p=3, depth=-1, ts=1, k="inv_r_reg"),
+    dict(id="3d_invr_d5", dim=3, nt=32768, ns=32768, shared=True, w="u01", p=3, depth=5, ts=1, k="inv_r_reg"),
+    dict(id="3d_invr_ts0", dim=3, nt=6000, ns=6000, shared=True, w="u01", p=3, depth=4, ts=0, k="inv_r_reg"),
+    dict(id="3d_invr_default", dim=3, nt=2000, ns=3000, shared=False, w="ones", p=2, depth=4, ts=0, k="inv_r"),
+    dict(id="3d_gauss_mix", dim=3, nt=20000, ns=20000, shared=False, w="normal", p=3, depth=-1, ts=1, k="gauss",
+         params=(0.05,), src="mixture"),
+    dict(id="3d_gauss_mix_d5", dim=3, nt=9000, ns=9000, shared=False, w="normal", p=3, depth=5, ts=1, k="gauss",
+         params=(0.05,), src="mixture"),
+    dict(id="3d_yukawa_p4", dim=3, nt=16384, ns=16384, shared=True, w="normal", p=4, depth=4, ts=1, k="yukawa",
+         params=(2.0,)),
+    dict(id="2d_sum_r2", dim=2, nt=5000, ns=4000, shared=False, w="u01", p=4, depth=5, ts=1, k="sum_r2"),
+    dict(id="2d_sum2_r2_general", dim=2, nt=3000, ns=2500, shared=False, w="u01", p=4, depth=4, ts=0, k="sum2_r2"),
+    dict(id="3d_sum2_r2_general", dim=3, nt=2000, ns=2000, shared=False, w="u01", p=2, depth=3, ts=0, k="sum2_r2"),
+    dict(id="3d_sum2_r2_sym", dim=3, nt=2000, ns=2000, shared=False, w="u01", p=2, depth=4, ts=1, k="sum2_r2"),
+    dict(id="1d_p0", dim=1, nt=700, ns=900, shared=False, w="pm1", p=0, depth=5, ts=1, k="inv_diff"),
+    dict(id="3d_p0_d3", dim=3, nt=700, ns=900, shared=False, w="u01", p=0, depth=3, ts=1, k="inv_r_reg"),
+    dict(id="3d_ragged", dim=3, nt=1001, ns=777, shared=False, w="u01", p=3, depth=4, ts=1, k="inv_r_reg"),
+    dict(id="2d_p7_d4", dim=2, nt=4000, ns=4000, shared=True, w="normal", p=7, depth=4, ts=1, k="log_r"),
+    dict(id="3d_p5_d3", dim=3, nt=3000, ns=3000, shared=True, w="u01", p=5, depth=3, ts=1, k="inv_r_reg"),
+]
+
+
+@pytest.mark.parametrize("case", FMM_CASES, ids=[c["id"] for c in FMM_CASES])
+def test_fmm_fp64_matches_oracle(gpu, oracle, case):
+    t, w, s = _inputs(case)
+    params = case.get("params", ())
+    print("case", case["id"], flush=True)
+    got, pl = gpu_solve(gpu, "fmm", t, w, s, case["p"], case["depth"], case["ts"], case["k"], params,
+                        shared=case["shared"])
+    ww = np.ones(s.shape[0]) if w is None else w
+    ref = oracle.fmm(t, ww, s, case["p"], pl.depth, case["ts"], KID[case["k"]], params)
+    assert np.all(np.isfinite(got)) == np.all(np.isfinite(ref))
+    fin = np.isfinite(ref)
+    err = rel_l2(got[fin], ref[fin])
+    print("  depth %d rel_l2 vs oracle %.3e" % (pl.depth, err))
+    assert err <= 1e-12, err
+
+
+def test_default_kernel_inf_at_coincident_points(gpu, oracle):
+    """Reference default kernel returns inf at r = 0 (include/fmma/fmma.hpp:55-63)."""
+    t = synth.points(1, 500, 2)
+    got, _ = gpu_solve(gpu, "fmm", t, None, t, 2, 3, 0, "inv_r", shared=True)
+    assert np.all(np.isinf(got))
+
+
+def test_shallow_depth_accumulates_into_ans(gpu, oracle):
+    """Leaf level < 2: no L2P, P2P adds onto ans (src/fmm.cpp:114-139)."""
+    t = synth.points(21, 600, 2)
+    s = synth.points(22, 500, 2)
+    w = synth.uniform(23, 500)
+    ans0 = synth.uniform(24, 600)
+    for depth in (1, 2):
+        got, _ = gpu_solve(gpu, "fmm", t, w, s, 3, depth, 1, "inv_r_reg", ans0=ans0)
+        ref = oracle.fmm(t, w, s, 3, depth, 1, 1, (), ans=ans0)
+        assert rel_l2(got, ref) <= 1e-13
+
+
+def test_depth_is_sticky(gpu):
+    """Depth auto-set on the first fmm call and kept (src/fmm.cpp:23-28)."""
+    t = synth.points(31, 5000, 3)
+    _, pl = gpu_solve(gpu, "fmm", t, None, t, 2, -1, 1, "inv_r_reg", shared=True)
+    assert pl.depth == int(np.log(5000) / 3) + 1
+    t2 = synth.points(32, 100000, 3)
+    _, pl = gpu_solve(gpu, "fmm", t2, None, t2, 2, -1, 1, "inv_r_reg", shared=True, plan=pl)
+    assert pl.depth == int(np.log(5000) / 3) + 1
+
+
+@pytest.mark.parametrize("dim,nt,ns,kernel,params", [
+    (1, 3000, 5000, "inv_diff", ()), (2, 2048, 3000, "log_r", ()), (3, 4096, 4096, "inv_r_reg", ()),
+    (3, 100, 200000, "yukawa", (2.0,)), (3, 257, 1000, "gauss", (0.05,)), (2, 333, 444, "sum2_r2", ()),
+])
+def test_exact_fp64_matches_oracle(gpu, oracle, dim, nt, ns, kernel, params):
+    t = synth.points(41, nt, dim)
+    s = synth.points(42, ns, dim)
+    w = synth.normal(43, ns)
+    got, _ = gpu_solve(gpu, "exact", t, w, s, kernel=kernel, params=params)
+    ref = oracle.exact(t, w, s, KID[kernel], params)
+    assert rel_l2(got, ref) <= 1e-13
+
+
+def test_exact_matvec_is_exact(gpu, oracle):
+    t = synth.points(44, 900, 2)
+    got, _ = gpu_solve(gpu, "exact_matvec", t, None, t, kernel="inv_r_reg", shared=True)
+    ref = oracle.exact(t, np.ones(900), t, 1)
+    assert rel_l2(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("dim,depth,dtype", [(1, 9, np.float64), (2, 6, np.float64), (3, 5, np.float64),
+                                             (3, 4, np.float32), (2, 7, np.float32)])
+def test_box_membership_bit_exact(gpu, oracle, dim, depth, dtype):
+    """Every point lands in the reference's leaf box (src/fmm_core.cpp:258-267)."""
+    n = 20000
+    t = synth.points(51, n, dim)
+    s = synth.points(52, n, dim)
+    NL = 1 << (depth - 1)
+    # points exactly on box faces and on the far boundary
+    s[:50] = np.floor(s[:50] * NL) / NL
+    s[50] = 1.0
+    t[0] = 0.0
+    t = t.astype(dtype)
+    s = s.astype(dtype)
+    _, pl = gpu_solve(gpu, "fmm", t, None, s, 1, depth, 1, "inv_r_reg", dtype=dtype)
+    for which, pts in (("s", s), ("t", t)):
+        got = pl.point_boxes(which, pts.shape[0])
+        ref = oracle.leaf_boxes(t, s, pts, depth)
+        assert np.array_equal(got, ref), which
+
+
+def test_shared_tree_membership(gpu, oracle):
+    t = synth.points(53, 10000, 3)
+    _, pl = gpu_solve(gpu, "fmm", t, None, t, 1, 5, 1, "inv_r_reg", shared=True)
+    assert np.array_equal(pl.point_boxes("s", 10000), oracle.leaf_boxes(t, t, t, 5))
+    assert np.array_equal(pl.point_boxes("t", 10000), oracle.leaf_boxes(t, t, t, 5))
+
+
+@pytest.mark.parametrize("dim,n,p,depth,ts", [(2, 6000, 4, 5, 1), (3, 12000, 3, 5, 1), (3, 3000, 2, 4, 0)])
+def test_stage_coefficients_match_oracle(gpu, oracle, dim, n, p, depth, ts):
+    """Per-level multipoles (P2M, M2M) and locals (M2L, L2L) at 1e-12."""
+    t = synth.points(61, n, dim)
+    s = synth.points(62, n, dim)
+    w = synth.uniform(63, n)
+    _, pl = gpu_solve(gpu, "fmm", t, w, s, p, depth, ts, "inv_r_reg")
+    _, Wm_ref, Wl_ref = oracle.fmm(t, w, s, p, depth, ts, 1, (), levels=True)
+    for l in range(2, depth):
+        assert rel_l2(pl.level_coeffs("Wm", l), Wm_ref[l]) <= 1e-12, ("Wm", l)
+        assert rel_l2(pl.level_coeffs("Wl", l), Wl_ref[l]) <= 1e-12, ("Wl", l)
+
+
+F32_CASES = [
+    dict(id="f32_3d", dim=3, nt=20000, ns=20000, shared=True, w="u01", p=3, depth=4, ts=1, k="inv_r_reg"),
+    dict(id="f32_2d_log", dim=2, nt=8000, ns=8000, shared=True, w="normal", p=5, depth=5, ts=1, k="log_r"),
+    dict(id="f32_3d_gauss", dim=3, nt=8000, ns=8000, shared=False, w="normal", p=3, depth=4, ts=1, k="gauss",
+         params=(0.05,), src="mixture"),
+]
+
+
+@pytest.mark.parametrize("case", F32_CASES, ids=[c["id"] for c in F32_CASES])
+def test_fmm_fp32(gpu, oracle, case):
+    t, w, s = _inputs(case)
+    t32 = t.astype(np.float32)
+    s32 = t32 if case["shared"] else s.astype(np.float32)
+    w32 = w.astype(np.float32)
+    params = case.get("params", ())
+    got, pl = gpu_solve(gpu, "fmm", t32, w32, s32, case["p"], case["depth"], case["ts"], case["k"], params,
+                        dtype=np.float32, shared=case["shared"])
+    ref = oracle.fmm(t32.astype(np.float64), w32.astype(np.float64), s32.astype(np.float64), case["p"], pl.depth,
+                     case["ts"], KID[case["k"]], params)
+    err = rel_l2(got, ref)
+    print("  f32 rel_l2 %.3e" % err)
+    assert err <= 3e-5, err
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("bf16x3", 2e-5)])
+def test_m2l_precision_forms(gpu, prec, tol):
+    """FP32-MFMA and split-BF16x3 M2L against the FP64 GPU fmm (precision error)."""
+    x = synth.points(71, 1 << 16, 3)
+    w = synth.uniform(72, 1 << 16)
+    ref, _ = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True)
+    got, _ = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True, precision=prec)
+    err = rel_l2(got, ref)
+    print("  %s precision error %.3e" % (prec, err))
+    assert err <= tol, err
+
+
+def test_solver_errors(gpu):
+    import fmma_amd
+    t = synth.points(81, 100, 2)
+    with pytest.raises(fmma_amd.FMMAError) as e:
+        gpu_solve(gpu, "nosuch", t, None, t)
+    assert e.value.code == 3
+    with pytest.raises(fmma_amd.FMMAError) as e:
+        gpu_solve(gpu, "tree", t, None, t)
+    assert e.value.code == 5
+
+
+def test_repeated_solves_are_deterministic(gpu):
+    x = synth.points(91, 30000, 3)
+    w = synth.uniform(92, 30000)
+    a, pl = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True)
+    b, _ = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True, plan=pl)
+    assert np.array_equal(a, b)
