@@ -1,0 +1,262 @@
+"""Benchmark: 3-D 1/r FMM solve, N = 4,194,304 points, on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-exact]
+
+A step is one full FMM solve (tree build, P2M, M2M, M2L, L2L, L2P, P2P) of
+the same synthetic workload: N points uniform in the unit cube (SplitMix64
+seed 1), targets = sources, weights U[0,1) (seed 2), f = 1/|x-y| with f = 0 at
+r = 0, 4 Chebyshev nodes per dimension (poly_ord 3), Depth 6 (the auto rule),
+trans_sym_flag 1.  Inputs are resident in HBM before the timed region.
+
+The headline `value` is ms per solve of the FP32-MFMA M2L form (P2P, geometry
+and operator set-up in FP64).  The same line reports the plain FP64 GPU form,
+the split-BF16x3 form, both error relations (precision error against the
+FP64 GPU fmm over all N targets; total error against the FP64 exact sum on
+8,192 sampled targets, computed on the GPU), per-stage times, the roofline of
+the dominant kernel (live HIP-event stage times), and the CPU baseline: the
+compiled reference (oracle/_ref) or its C restatement (oracle/) timed on this
+host on a bounded sample and scaled to N.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D N=4M FMM solve ms @ rel-err≤1e-6: FP32-MFMA M2L vs FP64 GPU vs CPU"
+PEAK_HBM = 8.0e12          # B/s
+PEAK_FP64 = 78.6e12        # FLOP/s, FP64 vector and matrix
+PEAK_FP32 = 157.3e12       # FLOP/s, FP32 matrix
+PEAK_BF16 = 2.5e15         # FLOP/s, dense bf16 matrix
+P2P_FLOPS_PER_PAIR = 12    # 3 sub, 3 mul, 2 add (r^2), sqrt, div (1/r), mul (w f), add (accumulate)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--depth", type=int, default=-1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--forms", default="fp32,native,bf16x3")
+    return ap.parse_args()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rms_rel(a, b):
+    return float(np.sqrt(np.mean(((a - b) / b) ** 2)))
+
+
+def pair_count(x_np, depth):
+    """Exact P2P pair interactions of the uniform tree (targets = sources):
+    sum over leaf boxes of n_b * (sources in the clipped 3^3 neighbourhood)."""
+    NL = 1 << (depth - 1)
+    lo, hi = x_np.min(0), x_np.max(0)
+    Len = float(np.max(hi - lo))
+    org = (hi + lo) / 2 - Len / 2
+    b = np.minimum(((x_np - org) / (Len / NL)).astype(np.int64), NL - 1)
+    cnt = np.zeros((NL, NL, NL), dtype=np.int64)
+    np.add.at(cnt, (b[:, 0], b[:, 1], b[:, 2]), 1)
+    pad = np.pad(cnt, 1)
+    nb = np.zeros_like(cnt)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                nb += pad[1 + dx:1 + dx + NL, 1 + dy:1 + dy + NL, 1 + dz:1 + dz + NL]
+    return int((cnt * nb).sum())
+
+
+def m2l_pairs(depth, dim=3):
+    return sum((6 * 2 ** l - 8) ** dim - (3 * 2 ** l - 2) ** dim for l in range(2, depth))
+
+
+def cpu_baseline():
+    """Time the reference fmm (compiled oracle/_ref if present, else the C
+    restatement) on N = 524,288 points (Depth 5, 128 points per leaf as at
+    N = 4M, Depth 6), trans_sym_flag 1, and scale by 8 to N = 4,194,304."""
+    from oracle import pyoracle
+    from fmma_amd import synth
+    n = 1 << 19
+    x = synth.points(1, n, 3)
+    w = synth.uniform(2, n)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    times = []
+    if pyoracle.have_ref():
+        kind = "reference"
+        pyoracle.ref_set_threads(cores)
+        for _ in range(2):
+            t0 = time.perf_counter()
+            pyoracle.ref_solve("fmm", x, w, x, 3, -1, 1, 1)
+            times.append(time.perf_counter() - t0)
+    else:
+        kind = "port"
+        for _ in range(2):
+            t0 = time.perf_counter()
+            pyoracle.fmm(x, w, x, 3, -1, 1, 1)
+            times.append(time.perf_counter() - t0)
+    ms = min(times) * 1e3
+    cpu_model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"value": round(ms * 8, 1), "unit": "ms (extrapolated x8 to N=4,194,304)", "cores": cores, "kind": kind,
+            "sample": "fmm solve, N=524,288 uniform points (Depth 5, 128 points/leaf), 4 nodes/dim, 1/r, "
+                      "trans_sym_flag 1; best of 2 runs = %.1f ms; scaled x8 (O(N))" % ms,
+            "cpu": cpu_model, "omp_threads": cores}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this FMM build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fmma_amd
+    from fmma_amd import synth
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    x_np = synth.points(1, n, 3)
+    w_np = synth.uniform(2, n)
+    x = torch.from_numpy(x_np).to(dev)
+    w = torch.from_numpy(w_np).to(dev)
+    torch.cuda.synchronize()
+
+    forms = [f for f in args.forms.split(",") if f]
+    plans, answers, results = {}, {}, {}
+    for form in forms:
+        pl = fmma_amd.Plan(3, "float64")
+        pl.set_solver_type("fmm")
+        pl.set_poly_ord(3)
+        pl.set_Depth(args.depth)
+        pl.set_trans_sym_flag(1)
+        pl.set_kernel("inv_r_reg")
+        pl.set_m2l_precision(form)
+        plans[form] = pl
+        answers[form] = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    def run(form, steps, warmup):
+        pl, a = plans[form], answers[form]
+        for _ in range(warmup):
+            pl.solve(x, w, x, a)
+        torch.cuda.synchronize()
+        pl.clear_time_log()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pl.solve(x, w, x, a)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        log = {k: v / steps for k, v in pl.time_log().items() if not k.startswith("total")}
+        return el * 1e3 / steps, log
+
+    for form in forms:
+        ms, log = run(form, args.steps, args.warmup)
+        results[form] = {"ms": round(ms, 3), "stages_ms": {k: round(v, 4) for k, v in log.items()}}
+        print("# form %-7s %.3f ms/solve  stages %s" % (form, ms, json.dumps(results[form]["stages_ms"])),
+              file=sys.stderr, flush=True)
+
+    depth = plans[forms[0]].depth
+    ref_form = "native" if "native" in forms else forms[0]
+    a_ref = answers[ref_form].cpu().numpy()
+    for form in forms:
+        if form != ref_form:
+            results[form]["precision_err_vs_fp64_fmm"] = rel_l2(answers[form].cpu().numpy(), a_ref)
+
+    # total error vs the FP64 exact sum on 8192 sampled targets (GPU exact solver)
+    if not args.no_exact:
+        idx = synth.sample_indices(3, n, 8192)
+        ts = x[torch.from_numpy(idx).to(dev)].contiguous()
+        ex = fmma_amd.Plan(3, "float64")
+        ex.set_solver_type("exact")
+        ex.set_kernel("inv_r_reg")
+        u = torch.zeros(len(idx), dtype=torch.float64, device=dev)
+        ex.solve(ts, w, x, u)
+        u = u.cpu().numpy()
+        for form in forms:
+            got = answers[form].cpu().numpy()[idx]
+            results[form]["total_err_vs_exact"] = rel_l2(got, u)
+            results[form]["rms_rel_err_vs_exact"] = rms_rel(got, u)
+
+    head = "fp32" if "fp32" in forms else forms[0]
+    ms_head = results[head]["ms"]
+    stages = results[head]["stages_ms"]
+    # roofline of the dominant kernel
+    pairs = pair_count(x_np, depth)
+    mpairs = m2l_pairs(depth)
+    dom = max(stages, key=stages.get)
+    P = 64
+    if dom == "P2P":
+        flops = pairs * P2P_FLOPS_PER_PAIR
+        ach = flops / (stages["P2P"] * 1e-3)
+        roof = {"kernel": "p2p_kernel", "bound": "valu-fp64", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
+                "work": "%d pair interactions x %d flops" % (pairs, P2P_FLOPS_PER_PAIR),
+                "pairs_per_s": pairs / (stages["P2P"] * 1e-3)}
+    elif dom == "M2L":
+        flops = 2 * P * P * mpairs
+        ach = flops / (stages["M2L"] * 1e-3)
+        peak = {"fp32": PEAK_FP32, "native": PEAK_FP64, "bf16x3": PEAK_BF16 / 3}[head]
+        roof = {"kernel": "m2l_mfma4", "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                "work": "2*P^2 x %d M2L pairs" % mpairs}
+    else:
+        nbytes = n * 4 * 8 + (8 ** (depth - 1)) * P * 8
+        ach = nbytes / (stages[dom] * 1e-3)
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "traffic": None}
+    for k in ("achieved", "peak", "frac"):
+        roof[k] = float("%.4g" % roof[k])
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": METRIC,
+        "value": ms_head,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_head,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (M2L %s)" % head,
+        "data": "synthetic",
+        "config": {"workload": "3-D 1/r FMM solve, N=%d uniform points in the unit cube, targets=sources, "
+                               "4 Chebyshev nodes/dim, Depth %d, trans_sym 1, M2L form %s" % (n, depth, head),
+                   "n": n, "poly_ord": 3, "depth": depth, "kernel": "1/|x-y| (0 at r=0)", "m2l": head,
+                   "parallelism": "single GPU"},
+        "forms": results,
+        "speedup_fp32_vs_fp64_gpu": round(results["native"]["ms"] / ms_head, 3) if "native" in results else None,
+        "p2p_pairs": pairs,
+        "m2l_pairs": mpairs,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu and "native" in results:
+        line["speedup_fp64_gpu_vs_cpu"] = round(cpu["value"] / results["native"]["ms"], 1)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
