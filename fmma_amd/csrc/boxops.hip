@@ -220,6 +220,301 @@ __global__ __launch_bounds__(64) void l2p_kernel(L2PArgs<T> a)
     }
 }
 
+// ======================================================================
+// Compile-time node count NN (2..8): the per-particle interpolation weights
+// live in registers or right-sized LDS, and M2M / L2L apply the 1-D
+// transfer matrices one dimension at a time (D*NN multiply-adds per
+// coefficient and child instead of NN^D).
+
+template <typename T, int NN>
+__device__ __forceinline__ void cheb_weights_ct(const T *coef, T r, T scale, T (&out)[NN])
+{
+    T tj[NN];
+    tj[0] = T(1);
+    if constexpr (NN > 1) tj[1] = r;
+#pragma unroll
+    for (int j = 2; j < NN; ++j) tj[j] = T(2) * r * tj[j - 1] - tj[j - 2];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+        T v = T(0);
+#pragma unroll
+        for (int j = 0; j < NN; ++j) v += coef[k * NN + j] * tj[j];
+        out[k] = v * scale;
+    }
+}
+
+constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+
+// P2M: one wave per source box, lane = coefficient (KPL per lane).
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
+{
+    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64;
+    __shared__ T sc[NN * NN];
+    __shared__ T sv[64][D * NN];
+    const unsigned box = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < NN * NN; i += 64) sc[i] = a.coef[i];
+    const unsigned s0 = a.start[box], s1 = a.start[box + 1];
+    int dig[KPL][D];
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        int k = lane + 64 * r;
+        k = k < P ? k : 0;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            dig[r][d] = k % NN;
+            k /= NN;
+        }
+    }
+    T acc[KPL];
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) acc[r] = T(0);
+    const int NL = 1 << a.leaf_level;
+    const T len = a.geom->len_leaf;
+    __syncthreads();
+    for (unsigned base = s0; base < s1; base += 64) {
+        const unsigned i = base + lane;
+        if (i < s1) {
+            const T wt = a.w[i];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                T rel;
+                leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+                T out[NN];
+                cheb_weights_ct<T, NN>(sc, rel, d == 0 ? wt : T(1), out);
+#pragma unroll
+                for (int k = 0; k < NN; ++k) sv[lane][d * NN + k] = out[k];
+            }
+        }
+        __syncthreads();
+        const int cnt = (int)(s1 - base < 64u ? s1 - base : 64u);
+        for (int p = 0; p < cnt; ++p) {
+#pragma unroll
+            for (int r = 0; r < KPL; ++r) {
+                T v = sv[p][dig[r][0]];
+#pragma unroll
+                for (int d = 1; d < D; ++d) v *= sv[p][d * NN + dig[r][d]];
+                acc[r] += v;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const int k = lane + 64 * r;
+        if (k < P) a.Wm[(size_t)box * P + k] = acc[r];
+    }
+}
+
+// out[..k_d..] = sum_c M[k_d][c] in[..c..] along dimension d of an NN^D
+// tensor (dimension 0 most significant); M is NN x NN (row = output index).
+template <typename T, int D, int NN>
+__device__ __forceinline__ void apply_dim(const T *M, bool transpose, int d, const T *in, T *out, int lane)
+{
+    constexpr int P = ipow_c(NN, D);
+    const int stride = d == D - 1 ? 1 : d == D - 2 ? NN : NN * NN;
+    for (int e = lane; e < P; e += 64) {
+        const int kd = (e / stride) % NN;
+        const int base = e - kd * stride;
+        T v = T(0);
+#pragma unroll
+        for (int c = 0; c < NN; ++c) v += (transpose ? M[c * NN + kd] : M[kd * NN + c]) * in[base + c * stride];
+        out[e] = v;
+    }
+}
+
+// M2M: one wave per parent.  For each child sigma the transfer
+// prod_d S_n(c_ko, (c_ki + 2 sigma_d - 1)/2) is applied dimension by
+// dimension through LDS and summed over the children in registers.
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(64) void m2m_ct(const T *Wc, T *Wp, const T *tab)
+{
+    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
+    __shared__ T st[2 * NN * NN];
+    __shared__ T sw[NC * P];
+    __shared__ T tA[P], tB[P];
+    const unsigned parent = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 2 * NN * NN; i += 64) st[i] = tab[i];
+    const T *src = Wc + (size_t)parent * NC * P;
+    for (int i = lane; i < NC * P; i += 64) sw[i] = src[i];
+    T acc[KPL];
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) acc[r] = T(0);
+    __syncthreads();
+    for (int sg = 0; sg < NC; ++sg) {
+        const T *cur = sw + sg * P;
+        // dimensions D-1 .. 1 through LDS
+        T *bufs[2] = {tA, tB};
+        int nb = 0;
+        for (int d = D - 1; d >= 1; --d) {
+            const int bit = (sg >> (D - 1 - d)) & 1;
+            apply_dim<T, D, NN>(st + bit * NN * NN, false, d, cur, bufs[nb], lane);
+            __syncthreads();
+            cur = bufs[nb];
+            nb ^= 1;
+        }
+        // dimension 0 into the accumulators
+        const T *M = st + ((sg >> (D - 1)) & 1) * NN * NN;
+        const int stride = ipow_c(NN, D - 1);
+#pragma unroll
+        for (int r = 0; r < KPL; ++r) {
+            const int e = lane + 64 * r;
+            if (e < P) {
+                const int k0 = e / stride, base = e - k0 * stride;
+                T v = T(0);
+#pragma unroll
+                for (int c = 0; c < NN; ++c) v += M[k0 * NN + c] * cur[base + c * stride];
+                acc[r] += v;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const int k = lane + 64 * r;
+        if (k < P) Wp[(size_t)parent * P + k] = acc[r];
+    }
+}
+
+// L2L: one wave per parent; each child gets the transposed separable transfer.
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab)
+{
+    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
+    __shared__ T st[2 * NN * NN];
+    __shared__ T sp[P], tA[P], tB[P];
+    const unsigned parent = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 2 * NN * NN; i += 64) st[i] = tab[i];
+    for (int i = lane; i < P; i += 64) sp[i] = Wp[(size_t)parent * P + i];
+    __syncthreads();
+    for (int sg = 0; sg < NC; ++sg) {
+        const T *cur = sp;
+        T *bufs[2] = {tA, tB};
+        int nb = 0;
+        for (int d = D - 1; d >= 1; --d) {
+            const int bit = (sg >> (D - 1 - d)) & 1;
+            apply_dim<T, D, NN>(st + bit * NN * NN, true, d, cur, bufs[nb], lane);
+            __syncthreads();
+            cur = bufs[nb];
+            nb ^= 1;
+        }
+        const T *M = st + ((sg >> (D - 1)) & 1) * NN * NN;
+        const int stride = ipow_c(NN, D - 1);
+        T *dst = Wc + ((size_t)parent * NC + sg) * P;
+#pragma unroll
+        for (int r = 0; r < KPL; ++r) {
+            const int e = lane + 64 * r;
+            if (e < P) {
+                const int k0 = e / stride, base = e - k0 * stride;
+                T v = T(0);
+#pragma unroll
+                for (int c = 0; c < NN; ++c) v += M[c * NN + k0] * cur[base + c * stride];
+                dst[e] += v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// L2P over sorted targets: workgroup of 256 threads = 256 consecutive
+// targets.  Their boxes form one contiguous Morton range (targets are
+// sorted by box); when that range is short its locals are staged in LDS,
+// otherwise they are read from global memory.
+constexpr int kL2PElems = 2048;  // LDS coefficients staged per workgroup
+
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
+{
+    constexpr int P = ipow_c(NN, D);
+    constexpr int NBX = kL2PElems / P > 0 ? kL2PElems / P : 1;
+    __shared__ T sc[NN * NN];
+    __shared__ T sW[NBX * P];
+    const long i0 = (long)blockIdx.x * 256;
+    const long i = i0 + threadIdx.x;
+    const long ilast = (i0 + 255 < a.n ? i0 + 255 : a.n - 1);
+    const unsigned kb0 = a.keys[i0], kb1 = a.keys[ilast];
+    const bool staged = kb1 - kb0 < (unsigned)NBX;
+    for (int e = threadIdx.x; e < NN * NN; e += 256) sc[e] = a.coef[e];
+    if (staged) {
+        const T *src = a.Wl + (size_t)kb0 * P;
+        const int cnt = (int)(kb1 - kb0 + 1) * P;
+        for (int e = threadIdx.x; e < cnt; e += 256) sW[e] = src[e];
+    }
+    __syncthreads();
+    if (i >= a.n) return;
+    const unsigned box = a.keys[i];
+    const T *W = staged ? sW + (size_t)(box - kb0) * P : a.Wl + (size_t)box * P;
+    const int NL = 1 << a.leaf_level;
+    const T len = a.geom->len_leaf;
+    T S[D][NN];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        T rel;
+        leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+        cheb_weights_ct<T, NN>(sc, rel, T(1), S[d]);
+    }
+    T acc = T(0);
+    if constexpr (D == 1) {
+#pragma unroll
+        for (int a0 = 0; a0 < NN; ++a0) acc += W[a0] * S[0][a0];
+    } else if constexpr (D == 2) {
+#pragma unroll
+        for (int a0 = 0; a0 < NN; ++a0)
+#pragma unroll
+            for (int a1 = 0; a1 < NN; ++a1) acc += W[a0 * NN + a1] * (S[0][a0] * S[1][a1]);
+    } else {
+#pragma unroll
+        for (int a0 = 0; a0 < NN; ++a0)
+#pragma unroll
+            for (int a1 = 0; a1 < NN; ++a1) {
+                const T s01 = S[0][a0] * S[1][a1];
+#pragma unroll
+                for (int a2 = 0; a2 < NN; ++a2) acc += W[(a0 * NN + a1) * NN + a2] * (s01 * S[2][a2]);
+            }
+    }
+    a.out[i] = acc;
+}
+
+#define FMMA_NN_SWITCH(NNV, CALL)                          \
+    switch (NNV) {                                         \
+    case 2: { constexpr int NN = 2; CALL; } return true;  \
+    case 3: { constexpr int NN = 3; CALL; } return true;  \
+    case 4: { constexpr int NN = 4; CALL; } return true;  \
+    case 5: { constexpr int NN = 5; CALL; } return true;  \
+    case 6: { constexpr int NN = 6; CALL; } return true;  \
+    case 7: { constexpr int NN = 7; CALL; } return true;  \
+    case 8: { constexpr int NN = 8; CALL; } return true;  \
+    default: return false;                                 \
+    }
+
+template <typename T, int D>
+static bool p2m_fast(const P2MArgs<T> &a, long nbox, hipStream_t st)
+{
+    FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((p2m_ct<T, D, NN>), dim3((unsigned)nbox), dim3(64), 0, st, a));
+}
+
+template <typename T, int D>
+static bool m2m_fast(const T *Wc, T *Wp, long nparent, const T *tab, int nn, hipStream_t st)
+{
+    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((m2m_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wc, Wp, tab));
+}
+
+template <typename T, int D>
+static bool l2l_fast(const T *Wp, T *Wc, long nparent, const T *tab, int nn, hipStream_t st)
+{
+    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((l2l_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wp, Wc, tab));
+}
+
+template <typename T, int D>
+static bool l2p_fast(const L2PArgs<T> &a, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((a.n + 255) / 256);
+    FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((l2p_ct<T, D, NN>), dim3(nb), dim3(256), 0, st, a));
+}
+
 // ------------------------------------------------------------------ launchers
 
 static int kpl_of(int P) { return P <= 64 ? 1 : P <= 128 ? 2 : P <= 256 ? 4 : 8; }
@@ -236,7 +531,8 @@ template <typename T, int D>
 int p2m(const P2MArgs<T> &a, long nbox, hipStream_t st)
 {
     if (nbox <= 0) return FMMA_OK;
-    FMMA_KPL_SWITCH(a.P, hipLaunchKernelGGL((p2m_kernel<T, D, KPL>), dim3((unsigned)nbox), dim3(64), 0, st, a));
+    if (!p2m_fast<T, D>(a, nbox, st))
+        FMMA_KPL_SWITCH(a.P, hipLaunchKernelGGL((p2m_kernel<T, D, KPL>), dim3((unsigned)nbox), dim3(64), 0, st, a));
     FMMA_LAUNCHED("p2m");
     return FMMA_OK;
 }
@@ -245,6 +541,7 @@ template <typename T, int D>
 int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st)
 {
     if (nparent <= 0) return FMMA_OK;
+    if (!m2m_fast<T, D>(Wc, Wp, nparent, tab, nn, st))
     FMMA_KPL_SWITCH(P, hipLaunchKernelGGL((m2m_kernel<T, D, KPL>), dim3((unsigned)nparent), dim3(64), 0, st, Wc,
                                           Wp, tab, nn, P));
     FMMA_LAUNCHED("m2m");
@@ -255,6 +552,7 @@ template <typename T, int D>
 int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st)
 {
     if (nparent <= 0) return FMMA_OK;
+    if (!l2l_fast<T, D>(Wp, Wc, nparent, tab, nn, st))
     FMMA_KPL_SWITCH(P, hipLaunchKernelGGL((l2l_kernel<T, D, KPL>), dim3((unsigned)nparent), dim3(64), 0, st, Wp,
                                           Wc, tab, nn, P));
     FMMA_LAUNCHED("l2l");
@@ -264,8 +562,8 @@ int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream
 template <typename T, int D>
 int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st)
 {
-    if (nbox <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((l2p_kernel<T, D>), dim3((unsigned)nbox), dim3(64), 0, st, a);
+    if (nbox <= 0 || a.n <= 0) return FMMA_OK;
+    if (!l2p_fast<T, D>(a, st)) hipLaunchKernelGGL((l2p_kernel<T, D>), dim3((unsigned)nbox), dim3(64), 0, st, a);
     FMMA_LAUNCHED("l2p");
     return FMMA_OK;
 }

@@ -21,6 +21,7 @@ struct P2MArgs {
 template <typename T>
 struct L2PArgs {
     const unsigned *start;  // [nbox+1] target ranges
+    const unsigned *keys;   // [n] leaf box (Morton) of each sorted target
     const T *pos;           // [D][n] sorted target coordinates
     long n;
     const Geom<T> *geom;

@@ -1,9 +1,14 @@
 // builtin_kernels.hpp -- device functors for the FMMA_KERNEL_* built-ins.
 //
 // Each follows the formula of the same id in oracle/fmm_oracle.c and
-// include/fmma_hip.h; 1/r uses a hardware reciprocal square root refined by
-// one third-order Newton (Householder) step, which leaves ~1 ulp error in
-// FP64: y0 = rsq(r2), e = 1 - r2 y0^2, y = y0 (1 + e/2 + 3e^2/8).
+// include/fmma_hip.h.  1/r uses the hardware reciprocal square root
+// (relative error below 5.3e-8 in FP64, measured on gfx950) refined by one
+// Newton step y = y0 (1 + e/2), e = 1 - r2 y0^2, which leaves at most 4.1e-15
+// (measured over 1e-8 <= r2 <= 1e2).
+//
+// Kernels that special-case r = 0 also provide far(): the same function
+// without that test, which P2P uses for the 3^D - 1 neighbour boxes other
+// than the target's own box (points in different boxes never coincide).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,7 +22,7 @@ __device__ __forceinline__ double rsqrt_refined(double r2)
     const double y = __builtin_amdgcn_rsq(r2);
     const double h = r2 * y;
     const double e = __builtin_fma(-h, y, 1.0);
-    return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
+    return __builtin_fma(y * e, 0.5, y);
 }
 
 __device__ __forceinline__ float rsqrt_refined(float r2)
@@ -49,6 +54,11 @@ struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.h
         const T r2 = dist2<T, D>(x, y);
         return r2 == T(0) ? T(INFINITY) : rsqrt_refined(r2);
     }
+    template <typename T, int D>
+    __device__ __forceinline__ T far(const T *x, const T *y) const
+    {
+        return rsqrt_refined(dist2<T, D>(x, y));
+    }
 };
 
 struct KInvRReg {  // 1/|x-y|, 0 at r = 0
@@ -60,6 +70,11 @@ struct KInvRReg {  // 1/|x-y|, 0 at r = 0
         const T r2 = dist2<T, D>(x, y);
         const T v = rsqrt_refined(r2);
         return r2 == T(0) ? T(0) : v;
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ T far(const T *x, const T *y) const
+    {
+        return rsqrt_refined(dist2<T, D>(x, y));
     }
 };
 
@@ -81,6 +96,11 @@ struct KLogR {  // log|x-y| = 0.5 log r^2, 0 at r = 0
     {
         const T r2 = dist2<T, D>(x, y);
         return r2 == T(0) ? T(0) : T(0.5) * std::log(r2);
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ T far(const T *x, const T *y) const
+    {
+        return T(0.5) * std::log(dist2<T, D>(x, y));
     }
 };
 
@@ -105,6 +125,13 @@ struct KYukawa {  // exp(-kappa r) / r, 0 at r = 0; p0 = kappa
         const T inv = rsqrt_refined(r2);
         const T v = std::exp(-(T)p0 * (r2 * inv)) * inv;
         return r2 == T(0) ? T(0) : v;
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ T far(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T inv = rsqrt_refined(r2);
+        return std::exp(-(T)p0 * (r2 * inv)) * inv;
     }
 };
 

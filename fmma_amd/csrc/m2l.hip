@@ -348,24 +348,232 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
     }
 }
 
+// Software-pipelined contraction for a compile-time padded length KP = 4*KQ.
+// Thread t gathers column t/4 (one target box's partner), elements
+// [(t%4)*KQ, +KQ), into registers for offset i+1 while the MFMAs of offset i
+// read LDS; the operator fragments of offset i+1 are prefetched the same way.
+template <typename TW, int D, int MODE, int KQ>
+__global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
+{
+    constexpr int KP = 4 * KQ;
+    constexpr int NKS = MODE == 2 ? KP / 32 : KP / 4;
+    using CT = typename std::conditional<MODE == 0, double, float>::type;
+    using V4 = typename std::conditional<MODE == 0, d4_t, f4_t>::type;
+    using ST = typename std::conditional<MODE == 2, __bf16, CT>::type;
+    using AT = typename std::conditional<MODE == 2, bf8_t, CT>::type;
+    constexpr int LDB = MODE == 2 ? KP + 8 : KP + 1;
+    constexpr int NPART = MODE == 2 ? 2 : 1;
+    __shared__ __align__(16) ST Bs[NPART * 64 * LDB];
+
+    const M2LTask task = a.tasks[blockIdx.x];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rb = blockIdx.y * 4 + wave;
+    const bool active = rb * 16 < a.MP;
+    const int N = 1 << task.level, P = a.P, nrb = a.MP / 16;
+    const int col = tid >> 2, part = tid & 3;
+
+    int tc[D];
+    const bool cvalid = col < task.np;
+    {
+        int pc[D];
+        morton_decode<D>(task.p0 + (cvalid ? col : 0), pc);
+#pragma unroll
+        for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((task.cls >> (D - 1 - d)) & 1);
+    }
+    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
+    const int *cls_f = a.cls_f + task.cls * a.n_cls;
+    const int *cls_o = a.cls_o + task.cls * a.n_cls * D;
+
+    TW g[KQ];
+    auto gather = [&](int i) {
+        int m[D];
+        bool ok = cvalid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m[d] = tc[d] + cls_o[i * D + d];
+            ok = ok && m[d] >= 0 && m[d] < N;
+        }
+        const TW *src = Wm + (size_t)(ok ? morton_encode<D>(m) : 0u) * P;
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) {
+            const int k = part * KQ + j;
+            g[j] = (ok && k < P) ? src[k] : TW(0);
+        }
+    };
+    AT afr[NKS];
+    auto load_a = [&](int i) {
+        if (!active) return;
+        const size_t f = (size_t)cls_f[i];
+        if constexpr (MODE == 2) {
+            const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + a.kf_off[task.level]);
+            const size_t base = (f * nrb + rb) * NKS * 64;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) afr[s] = Kh[base + s * 64 + lane];
+        } else {
+            const CT *Kf = static_cast<const CT *>(a.Kf) + a.kf_off[task.level];
+            const size_t base = (f * nrb + rb) * NKS * 64;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) afr[s] = Kf[base + s * 64 + lane];
+        }
+    };
+    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKS : 1];
+    auto load_a_lo = [&](int i) {
+        if constexpr (MODE == 2) {
+            if (!active) return;
+            const size_t f = (size_t)cls_f[i];
+            const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
+            const size_t base = (f * nrb + rb) * NKS * 64;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) afr_lo[s] = Kl[base + s * 64 + lane];
+        }
+    };
+
+    TW acc[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[c][q] = TW(0);
+
+    const int n = a.n_cls;
+    gather(0);
+    load_a(0);
+    load_a_lo(0);
+    for (int i = 0; i < n; ++i) {
+        // registers -> LDS tile of offset i
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) {
+            const int k = part * KQ + j;
+            if constexpr (MODE == 2) {
+                const float v = (float)g[j];
+                const __bf16 h = (__bf16)v;
+                Bs[col * LDB + k] = h;
+                Bs[64 * LDB + col * LDB + k] = (__bf16)(v - (float)h);
+            } else {
+                Bs[col * LDB + k] = (CT)g[j];
+            }
+        }
+        __syncthreads();
+        AT acur[NKS];
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) acur[s] = afr[s];
+        [[maybe_unused]] AT acur_lo[MODE == 2 ? NKS : 1];
+        if constexpr (MODE == 2) {
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) acur_lo[s] = afr_lo[s];
+        }
+        if (i + 1 < n) {
+            gather(i + 1);
+            load_a(i + 1);
+            load_a_lo(i + 1);
+        }
+        if (active) {
+            V4 c[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) c[cb] = V4{0, 0, 0, 0};
+            if constexpr (MODE == 2) {
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    const int kk = 32 * s + 8 * (lane >> 4);
+#pragma unroll
+                    for (int cb = 0; cb < 4; ++cb) {
+                        const int off = (16 * cb + (lane & 15)) * LDB + kk;
+                        const bf8_t bh = *reinterpret_cast<const bf8_t *>(Bs + off);
+                        const bf8_t bl = *reinterpret_cast<const bf8_t *>(Bs + 64 * LDB + off);
+                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[s], bh, c[cb], 0, 0, 0);
+                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bl, c[cb], 0, 0, 0);
+                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bh, c[cb], 0, 0, 0);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    const int kk = 4 * s + (lane >> 4);
+#pragma unroll
+                    for (int cb = 0; cb < 4; ++cb) {
+                        const CT bv = Bs[(16 * cb + (lane & 15)) * LDB + kk];
+                        if constexpr (MODE == 0)
+                            c[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[s], bv, c[cb], 0, 0, 0);
+                        else
+                            c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[s], bv, c[cb], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[cb][q] += (TW)c[cb][q];
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int cc = 16 * cb + (lane & 15);
+        if (cc >= task.np) continue;
+        const size_t box = ((size_t)(task.p0 + cc) << D) | (unsigned)task.cls;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
+            if (row < P) Wl[box * P + row] = acc[cb][q];
+        }
+    }
+}
+
+template <typename T, int D, int MODE, int KQ>
+static bool launch_pipe_kq(const M2LKArgs &a, dim3 grid, hipStream_t st)
+{
+    if constexpr (MODE == 2 && KQ % 8 != 0) {
+        return false;  // split-bf16 fragments need KP a multiple of 32
+    } else {
+        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KQ>), grid, dim3(256), 0, st, a);
+        return true;
+    }
+}
+
+template <typename T, int D, int MODE>
+static bool launch_pipe(const M2LKArgs &a, dim3 grid, hipStream_t st)
+{
+    switch (a.KP / 4) {
+    case 2: return launch_pipe_kq<T, D, MODE, 2>(a, grid, st);
+    case 4: return launch_pipe_kq<T, D, MODE, 4>(a, grid, st);
+    case 7: return launch_pipe_kq<T, D, MODE, 7>(a, grid, st);
+    case 8: return launch_pipe_kq<T, D, MODE, 8>(a, grid, st);
+    case 9: return launch_pipe_kq<T, D, MODE, 9>(a, grid, st);
+    case 16: return launch_pipe_kq<T, D, MODE, 16>(a, grid, st);
+    case 32: return launch_pipe_kq<T, D, MODE, 32>(a, grid, st);
+    }
+    return false;
+}
+
 template <typename T, int D>
 int m2l_apply(const M2LKArgs &a, int n_tasks, int mode, hipStream_t st)
 {
     if (n_tasks <= 0) return FMMA_OK;
     const dim3 grid((unsigned)n_tasks, (unsigned)((a.MP + 63) / 64));
+    bool done = false;
     if (mode == 0) {
         if constexpr (std::is_same<T, double>::value) {
-            const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
-            hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
+            done = launch_pipe<T, D, 0>(a, grid, st);
+            if (!done) {
+                const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
+                hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
+            }
         } else {
             return FMMA_ERR_INVALID;
         }
     } else if (mode == 1) {
-        const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
-        hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
+        done = launch_pipe<T, D, 1>(a, grid, st);
+        if (!done) {
+            const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
+            hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
+        }
     } else {
-        const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);
-        hipLaunchKernelGGL((m2l_bf16x3<T, D>), grid, dim3(256), lds, st, a);
+        done = launch_pipe<T, D, 2>(a, grid, st);
+        if (!done) {
+            const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);
+            hipLaunchKernelGGL((m2l_bf16x3<T, D>), grid, dim3(256), lds, st, a);
+        }
     }
     FMMA_LAUNCHED("m2l_apply");
     return FMMA_OK;

@@ -261,6 +261,7 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
         return rc;
     const T *t_pos = pl.s_pos.as<T>();
     const unsigned *t_start = pl.s_start.as<unsigned>(), *t_perm = pl.s_vals.as<unsigned>();
+    const unsigned *t_keys = pl.s_keys.as<unsigned>();
     if (!shared) {
         if ((rc = build_tree<T, D>(pl, tg, nullptr, nt, L, false, pl.t_keys, pl.t_vals, pl.t_keys_alt,
                                    pl.t_vals_alt, pl.t_hist, pl.t_pos, nullptr, pl.t_start)))
@@ -268,6 +269,7 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
         t_pos = pl.t_pos.as<T>();
         t_start = pl.t_start.as<unsigned>();
         t_perm = pl.t_vals.as<unsigned>();
+        t_keys = pl.t_keys.as<unsigned>();
     }
     pl.mark(1);
 
@@ -331,7 +333,7 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
     const T *base = nullptr;
     if (L >= 2) {
         if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
-        L2PArgs<T> la{t_start, t_pos, nt, pl.geom.as<Geom<T>>(), L, pl.d_coef.as<T>(), n, P, Wl + pl.lvl_off[L],
+        L2PArgs<T> la{t_start, t_keys, t_pos, nt, pl.geom.as<Geom<T>>(), L, pl.d_coef.as<T>(), n, P, Wl + pl.lvl_off[L],
                       pl.l2p_out.as<T>()};
         if ((rc = l2p<T, D>(la, nbox, st))) return rc;
         base = pl.l2p_out.as<T>();

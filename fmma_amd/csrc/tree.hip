@@ -174,42 +174,88 @@ __global__ __launch_bounds__(256) void rs_hist(const unsigned *keys, long n, int
     hist[(long)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive prefix sum of m values in place, one workgroup of 1024 threads
-// walking the array in chunks of 4096 with a running carry.
-__global__ __launch_bounds__(1024) void scan_exclusive(unsigned *a, long m)
+// Exclusive prefix sum of m values in place, in three launches: per-chunk
+// sums, a scan of those sums, then a scan of every chunk from its offset.
+constexpr int SCAN_BLOCKS = 256;
+
+// Inclusive scan of v over a workgroup of 256 threads; returns this thread's
+// inclusive value and the block total.
+__device__ __forceinline__ unsigned block_scan256(unsigned v, unsigned &total, unsigned *wsum)
 {
-    __shared__ unsigned wsum[16];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    unsigned carry = 0;
-    for (long base = 0; base < m; base += 4096) {
-        const long i0 = base + tid * 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    unsigned before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const unsigned ws = wsum[w];
+        if (w < wave) before += ws;
+        total += ws;
+    }
+    __syncthreads();
+    return before + x;
+}
+
+__global__ __launch_bounds__(256) void scan_reduce(const unsigned *a, long m, long chunk, unsigned *part)
+{
+    __shared__ unsigned wsum[4];
+    const long b0 = (long)blockIdx.x * chunk, b1 = b0 + chunk < m ? b0 + chunk : m;
+    unsigned s = 0;
+    for (long i = b0 + threadIdx.x; i < b1; i += 256) s += a[i];
+    unsigned total;
+    block_scan256(s, total, wsum);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void scan_parts(unsigned *part, int np)
+{
+    __shared__ unsigned wsum[4];
+    const unsigned v = (int)threadIdx.x < np ? part[threadIdx.x] : 0u;
+    unsigned total;
+    const unsigned inc = block_scan256(v, total, wsum);
+    if ((int)threadIdx.x < np) part[threadIdx.x] = inc - v;
+}
+
+__global__ __launch_bounds__(256) void scan_down(unsigned *a, long m, long chunk, const unsigned *part)
+{
+    __shared__ unsigned wsum[4];
+    const long b0 = (long)blockIdx.x * chunk, b1 = b0 + chunk < m ? b0 + chunk : m;
+    unsigned carry = part[blockIdx.x];
+    for (long base = b0; base < b1; base += 1024) {
+        const long i0 = base + threadIdx.x * 4;
         unsigned v[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = i0 + q < m ? a[i0 + q] : 0u;
+        for (int q = 0; q < 4; ++q) v[q] = i0 + q < b1 ? a[i0 + q] : 0u;
         const unsigned s = v[0] + v[1] + v[2] + v[3];
-        unsigned x = s;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned y = __shfl_up(x, o);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        unsigned before = carry, total = 0;
-        for (int w = 0; w < 16; ++w) {
-            const unsigned ws = wsum[w];
-            if (w < wave) before += ws;
-            total += ws;
-        }
-        unsigned e = before + x - s;
+        unsigned total;
+        const unsigned inc = block_scan256(s, total, wsum);
+        unsigned e = carry + inc - s;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if (i0 + q < m) a[i0 + q] = e;
+            if (i0 + q < b1) a[i0 + q] = e;
             e += v[q];
         }
         carry += total;
-        __syncthreads();
     }
+}
+
+static int scan_exclusive(unsigned *a, long m, unsigned *part, hipStream_t st)
+{
+    long chunk = (m + SCAN_BLOCKS - 1) / SCAN_BLOCKS;
+    chunk = (chunk + 1023) / 1024 * 1024;
+    const int nb = (int)((m + chunk - 1) / chunk);
+    hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(256), 0, st, a, m, chunk, part);
+    hipLaunchKernelGGL(scan_parts, dim3(1), dim3(256), 0, st, part, nb);
+    hipLaunchKernelGGL(scan_down, dim3(nb), dim3(256), 0, st, a, m, chunk, part);
+    FMMA_LAUNCHED("scan_exclusive");
+    return FMMA_OK;
 }
 
 __global__ __launch_bounds__(256) void rs_scatter(const unsigned *kin, const unsigned *vin, unsigned *kout,
@@ -276,7 +322,8 @@ int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st)
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
-        hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, st, b.hist, (long)ntiles * 256);
+        int rc = scan_exclusive(b.hist, (long)ntiles * 256, b.hist + radix_hist_words(n) - SCAN_BLOCKS, st);
+        if (rc) return rc;
         hipLaunchKernelGGL(rs_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, n, shift, b.hist,
                            ntiles);
         FMMA_LAUNCHED("radix_sort");
@@ -300,7 +347,7 @@ int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st)
 size_t radix_hist_words(long n)
 {
     const long ntiles = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)(ntiles > 0 ? ntiles : 1) * 256;
+    return (size_t)(ntiles > 0 ? ntiles : 1) * 256 + SCAN_BLOCKS;  // + scan partials
 }
 
 // --------------------------------------------------------------- box ranges

@@ -15,80 +15,204 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "../../fmma_hip.h"
 #include "morton.hpp"
 
 namespace fmma {
 namespace detail {
 
+// A functor may provide far(x, y): f for points known to be distinct (in
+// different leaf boxes), without its r = 0 special case.
+template <class K, typename T, int D, typename = void>
+struct has_far : std::false_type {};
+template <class K, typename T, int D>
+struct has_far<K, T, D,
+               std::void_t<decltype(std::declval<const K &>().template far<T, D>((const T *)nullptr,
+                                                                                  (const T *)nullptr))>>
+    : std::true_type {};
+
+template <bool FAR, typename T, int D, class K>
+__device__ __forceinline__ T pair_eval(const K &k, const T *x, const T *y)
+{
+    if constexpr (FAR && has_far<K, T, D>::value)
+        return k.template far<T, D>(x, y);
+    else
+        return k.template operator()<T, D>(x, y);
+}
+
 // ----------------------------------------------------------------- P2P
 //
-// One workgroup of 256 threads per target leaf box (Morton order).  Wave w
-// takes targets [start + 64w, +64), then +256, ...; every lane keeps one
-// target in registers and the wave streams the sources of the 3^D clipped
-// neighbour boxes in the reference's order (row-major over the block, each
-// box in ascending source index).  The source index is wave-uniform, so the
-// coordinates and weights arrive through scalar loads, with no LDS and no
-// barrier.  The result is written once, in the caller's target order.
+// One workgroup of 256 threads per target leaf box (Morton order).  Thread t
+// is (slot s = t % 16, slice q = t / 16): slot s owns the box's targets
+// s, s+16, s+32, ... (up to kP2PTargets of them in registers per pass) and
+// slice q takes every 16th source of the neighbourhood.  The sources of the
+// 3^D clipped neighbour boxes (src/fmm_core.cpp:81-115), concatenated, are
+// staged through LDS in blocks of kP2PBlock as (x.., w) records, so every
+// source is read from memory once per box and from LDS by 16 threads at a
+// time.  Work is split evenly over all 256 threads whatever the box's target
+// count (granularity 16 targets), and each LDS read feeds up to 12 pair
+// evaluations.  The 16 slice partial sums of a target are added in a fixed
+// order (wave shuffles, then LDS), so results are deterministic.
+constexpr int kP2PTargets = 12;  // targets per thread per pass
+constexpr int kP2PBlock = 512;   // sources per LDS stage
+
+template <int D>
+struct p2p_rec {  // LDS record width in elements: coordinates + weight, padded to even
+    static constexpr int W = (D + 2) & ~1;
+};
+
+// The pair loop over staged records [j0, j1) of slice q (indices = q mod 16)
+// for a compile-time number of targets.
+template <int TPL, bool FAR, typename T, int D, class K>
+__device__ __forceinline__ void p2p_range(const T *src, unsigned j0, unsigned j1, unsigned q,
+                                          const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
+{
+    constexpr int RW = p2p_rec<D>::W;
+    for (unsigned j = j0 + ((q + 16u - (j0 & 15u)) & 15u); j < j1; j += 16) {
+        const T *r = src + j * RW;
+        T y[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) y[d] = r[d];
+        const T wj = r[D];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) acc[k] += wj * pair_eval<FAR, T, D>(kern, x[k], y);
+    }
+}
+
+// Records [0, nself) of the block come from the target's own box and may
+// coincide with a target; the rest cannot.
+template <int TPL, typename T, int D, class K>
+__device__ __forceinline__ void p2p_block(const T *src, unsigned cnt, unsigned nself, unsigned q,
+                                          const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
+{
+    const unsigned m = nself < cnt ? nself : cnt;
+    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, x, acc, kern);
+    p2p_range<TPL, true>(src, m, cnt, q, x, acc, kern);
+}
+
 template <typename T, int D, class K>
 __global__ __launch_bounds__(256) void p2p_kernel(fmma_p2p_args a, K kern)
 {
+    constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
+    constexpr int RW = p2p_rec<D>::W;
+    __shared__ __align__(16) T src[kP2PBlock * RW];
+    __shared__ T red[4][16][kP2PTargets];
+    __shared__ unsigned nb_first[NNB], nb_pre[NNB + 1];
+
     const unsigned box = blockIdx.x;
     const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
-    const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    unsigned tb = t0 + wave * 64u;
-    if (tb >= t1) return;
-
-    int bc[D], lo[D], hi[D];
-    morton_decode<D>(box, bc);
-    const int NL = 1 << a.leaf_level;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        lo[d] = bc[d] > 0 ? bc[d] - 1 : bc[d];
-        hi[d] = bc[d] + 1 < NL ? bc[d] + 1 : bc[d];
-    }
+    if (t0 == t1) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s = lane & 15;
+    const unsigned q = (unsigned)tid >> 4;
     const T *tp = static_cast<const T *>(a.t_pos);
     const T *sp = static_cast<const T *>(a.s_pos);
     const T *sw = static_cast<const T *>(a.s_w);
     const long nt = a.n_t, ns = a.n_s;
 
-    for (; tb < t1; tb += 256u) {
-        const unsigned ti = tb + lane;
-        const bool valid = ti < t1;
-        const unsigned tc = valid ? ti : t1 - 1;
-        T x[D];
+    // neighbour ranges: the box itself first, then the other entries of the
+    // row-major 3^D block (last dimension fastest)
+    if (tid < NNB) {
+        int bc[D], m[D];
+        morton_decode<D>(box, bc);
+        const int NL = 1 << a.leaf_level;
+        bool ok = true;
+        constexpr int centre = (NNB - 1) / 2;
+        int k = tid == 0 ? centre : tid <= centre ? tid - 1 : tid;
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = tp[d * nt + tc];
-        T acc = T(0);
-        int nb[D];
+        for (int d = D - 1; d >= 0; --d) {
+            m[d] = bc[d] + (k % 3) - 1;
+            k /= 3;
+            ok = ok && m[d] >= 0 && m[d] < NL;
+        }
+        unsigned first = 0, cnt = 0;
+        if (ok) {
+            const unsigned mm = morton_encode<D>(m);
+            first = a.s_start[mm];
+            cnt = a.s_start[mm + 1] - first;
+        }
+        nb_first[tid] = first;
+        nb_pre[tid + 1] = cnt;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        nb_pre[0] = 0;
+        for (int k = 0; k < NNB; ++k) nb_pre[k + 1] += nb_pre[k];
+    }
+    __syncthreads();
+    const unsigned total = nb_pre[NNB], nself = nb_pre[1];
+
+    for (unsigned pass0 = t0; pass0 < t1; pass0 += 16 * kP2PTargets) {
+        const unsigned npass = t1 - pass0 < 16u * kP2PTargets ? t1 - pass0 : 16u * kP2PTargets;
+        const int tpl = (int)((npass + 15) / 16);
+        T x[kP2PTargets][D], acc[kP2PTargets];
 #pragma unroll
-        for (int d = 0; d < D; ++d) nb[d] = lo[d];
-        while (true) {
-            const unsigned m = morton_encode<D>(nb);
-            const unsigned s0 = a.s_start[m], s1 = a.s_start[m + 1];
-#pragma unroll 4
-            for (unsigned j = s0; j < s1; ++j) {
-                T y[D];
+        for (int k = 0; k < kP2PTargets; ++k) {
+            acc[k] = T(0);
+            const unsigned i = pass0 + s + 16u * k;
+            const unsigned ic = i < t1 ? i : t1 - 1;
 #pragma unroll
-                for (int d = 0; d < D; ++d) y[d] = sp[d * ns + j];
-                acc += sw[j] * kern.template operator()<T, D>(x, y);
+            for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
+        }
+        for (unsigned base = 0; base < total; base += kP2PBlock) {
+            const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
+            __syncthreads();  // previous block fully consumed
+            for (unsigned e = tid; e < cnt; e += 256) {
+                const unsigned p = base + e;
+                int b = 0;
+                while (nb_pre[b + 1] <= p) ++b;
+                const long j = (long)nb_first[b] + (p - nb_pre[b]);
+                T *r = src + e * RW;
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = sp[d * ns + j];
+                r[D] = sw[j];
             }
-            // row-major odometer over the neighbour block, last dimension fastest
-            int d = D - 1;
-            while (d >= 0 && nb[d] == hi[d]) { nb[d] = lo[d]; --d; }
-            if (d < 0) break;
-            ++nb[d];
+            __syncthreads();
+            const unsigned ns_blk = nself > base ? nself - base : 0u;
+            switch (tpl) {
+            case 1: p2p_block<1>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 2: p2p_block<2>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 3: p2p_block<3>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 4: p2p_block<4>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 5: p2p_block<5>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 6: p2p_block<6>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 7: p2p_block<7>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 8: p2p_block<8>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 9: p2p_block<9>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 10: p2p_block<10>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 11: p2p_block<11>(src, cnt, ns_blk, q, x, acc, kern); break;
+            default: p2p_block<12>(src, cnt, ns_blk, q, x, acc, kern); break;
+            }
         }
-        if (valid) {
+        // sum the 16 slices: q = 4*wave + lane/16
+#pragma unroll
+        for (int k = 0; k < kP2PTargets; ++k) {
+            if (k < tpl) {
+                T v = acc[k];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (lane < 16) red[wave][s][k] = v;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < 16 * tpl; e += 256) {
+            const int ss = e & 15, k = e >> 4;
+            const unsigned i = pass0 + ss + 16u * k;
+            if (i >= t1) continue;
+            const T v = red[0][ss][k] + red[1][ss][k] + red[2][ss][k] + red[3][ss][k];
             T *ans = static_cast<T *>(a.ans);
-            const unsigned oi = a.t_perm[ti];
+            const unsigned oi = a.t_perm[i];
             if (a.base)
-                ans[oi] = static_cast<const T *>(a.base)[ti] + acc;
+                ans[oi] = static_cast<const T *>(a.base)[i] + v;
             else if (a.accumulate)
-                ans[oi] += acc;
+                ans[oi] += v;
             else
-                ans[oi] = acc;
+                ans[oi] = v;
         }
+        __syncthreads();  // red[] is reused by the next pass
     }
 }
 
