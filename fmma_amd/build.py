@@ -37,6 +37,37 @@ def _compile(src):
     return obj
 
 
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s failed:\n%s" % (what, r.stderr[-6000:]))
+
+
+def build_pyfmma():
+    """The pybind11 drop-in module fmma_amd/pyfmma*.so (host C++ over the C-ABI)."""
+    import sysconfig
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(HERE, "pyfmma" + suffix)
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "pyfmma.cpp"), "-L" + HERE, "-lfmma_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd, "pyfmma build")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cpp_tests():
+    """tests/cpp/dropin_test: fmma::FMMA used as the reference's C++ users use it."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+    cmd = [HIPCC] + HIP_FLAGS + [src, "-L" + HERE, "-lfmma_hip", "-Wl,-rpath,$ORIGIN/../../fmma_amd",
+                                 "-o", out + ".tmp"]
+    _run(cmd, "dropin_test build")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(SOURCES), max(1, (os.cpu_count() or 4)))
@@ -48,8 +79,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
+    extra = [build_pyfmma(), build_cpp_tests()]
     if verbose:
-        print("built", LIB)
+        print("built", LIB, *extra)
     return LIB
 
 

@@ -1,0 +1,145 @@
+// Drop-in check of fmma::FMMA<TYPE,DIM> (include/fmma/fmma.hpp) used the way
+// the reference's own tests and tutorials use it (test/src/fmma/fmm.cpp,
+// tutorials/tutorial0*/): std::vector<std::array> inputs, set_fn with a
+// functor, set_solver_type, solve.  Built with hipcc; run on the GPU by
+// tests/test_gpu_dropin.py.  Prints one "name value" line per check and exits
+// non-zero on a failed check.
+#include <fmma/fmma.hpp>
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+static uint64_t sm_state = 0;
+static double uni()  // SplitMix64 uniform in [0, 1)
+{
+    uint64_t z = (sm_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T, std::size_t D>
+static std::vector<std::array<T, D>> pts(size_t n)
+{
+    std::vector<std::array<T, D>> v(n);
+    for (auto &p : v)
+        for (auto &c : p) c = (T)uni();
+    return v;
+}
+
+template <typename T>
+static double rel(const std::vector<T> &a, const std::vector<T> &b)
+{
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        num += ((double)a[i] - b[i]) * ((double)a[i] - b[i]);
+        den += (double)b[i] * b[i];
+    }
+    return std::sqrt(num / den);
+}
+
+static int fails = 0;
+static void check(const char *name, double v, double tol)
+{
+    printf("%s %.3e\n", name, v);
+    if (!(v <= tol)) {
+        printf("FAIL %s > %.1e\n", name, tol);
+        ++fails;
+    }
+}
+
+int main()
+{
+    sm_state = 7;
+    {  // 3-D, one-argument f(x - y) lambda vs the built-in 1/r kernel
+        const size_t n = 20000;
+        auto x = pts<double, 3>(n);
+        std::vector<double> w(n);
+        for (auto &v : w) v = uni();
+        fmma::FMMA<double, 3> f;
+        f.set_fn([] __host__ __device__(const std::array<double, 3> &d) {
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            return r2 == 0.0 ? 0.0 : 1.0 / std::sqrt(r2);
+        });
+        f.set_solver_type("fmm");
+        f.set_poly_ord(3);
+        f.set_Depth(4);
+        std::vector<double> a, b, e;
+        f.solve(x, w, x, a);
+        fmma::FMMA<double, 3> g;
+        g.set_kernel(FMMA_KERNEL_INV_R_REG);
+        g.set_solver_type("fmm");
+        g.set_poly_ord(3);
+        g.set_Depth(4);
+        g.set_trans_sym_flag(1);
+        g.solve(x, w, x, b);
+        f.set_solver_type("exact");
+        f.solve(x, w, x, e);
+        check("lambda_vs_builtin_3d", rel(a, b), 1e-12);
+        check("fmm_vs_exact_3d", rel(a, e), 1e-3);
+        check("depth_kept", std::abs(f.get_Depth() - 4), 0);
+    }
+    {  // 2-D two-argument f(x, y), not translation invariant (test/src/fmma/fmm.cpp:151-161)
+        const size_t n = 3000;
+        auto t = pts<double, 2>(n), s = pts<double, 2>(n);
+        std::vector<double> w(n);
+        for (auto &v : w) v = uni();
+        fmma::FMMA<double, 2> f;
+        f.set_fn([] __host__ __device__(const std::array<double, 2> &x, const std::array<double, 2> &y) {
+            double l2 = 0, s2 = 0;
+            for (int d = 0; d < 2; ++d) {
+                l2 += (x[d] - y[d]) * (x[d] - y[d]);
+                s2 += (x[d] + y[d]) * (x[d] + y[d]);
+            }
+            return s2 / l2;
+        });
+        f.set_solver_type("fmm");
+        f.set_poly_ord(5);
+        f.set_Depth(4);
+        std::vector<double> a, e, b;
+        f.fmm(t, w, s, a);
+        f.exact(t, w, s, e);
+        fmma::FMMA<double, 2> g;
+        g.set_kernel(FMMA_KERNEL_SUM2_R2);
+        g.set_poly_ord(5);
+        g.set_Depth(4);
+        g.fmm(t, w, s, b);
+        check("general_fmm_vs_exact_2d", rel(a, e), 1e-3);
+        check("general_lambda_vs_builtin_2d", rel(a, b), 1e-12);
+    }
+    {  // 1-D float, unweighted, the reference's README usage (README.md:75-78)
+        const size_t n = 4096;
+        auto t = pts<float, 1>(n), s = pts<float, 1>(n);
+        fmma::FMMA<float, 1> f;
+        f.set_fn([] __host__ __device__(const std::array<float, 1> &x, const std::array<float, 1> &y) {
+            return 1.0f / (x[0] - y[0]);
+        });
+        f.set_trans_sym_flag(1);
+        f.set_solver_type("fmm");
+        f.set_poly_ord(7);
+        std::vector<float> a, e;
+        f.solve(t, s, a);
+        f.set_solver_type("exact");
+        f.solve(t, s, e);
+        check("float_1d_fmm_vs_exact", rel(a, e), 1e-3);
+    }
+    {  // time log keys of the reference (src/fmm.cpp:57-141, src/fmma.cpp:189)
+        fmma::FMMA<double, 2> f;
+        auto x = pts<double, 2>(5000);
+        std::vector<double> a;
+        f.set_solver_type("fmm");
+        f.set_kernel(FMMA_KERNEL_LOG_R);
+        f.set_poly_ord(4);
+        f.solve(x, x, a);
+        auto log = f.get_time_log();
+        int have = 0;
+        for (const char *k : {"P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "total time(fmm)"}) have += log.count(k);
+        check("time_log_missing_keys", 7 - have, 0);
+    }
+    printf(fails ? "FAILED\n" : "OK\n");
+    return fails ? 1 : 0;
+}
