@@ -78,15 +78,32 @@ void OffsetTables::build(int dim)
     }
 }
 
-std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level)
+// Tasks level by level, deepest first.  A level's tile width is the widest
+// of 256/128/64 boxes that still gives >= 512 workgroups (two per CU), when a
+// pipelined kernel exists for it; tasks of equal width form one launch.
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups)
 {
     std::vector<M2LTask> t;
+    groups.clear();
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
     for (int l = leaf_level; l >= 2; --l) {
         const unsigned npar = 1u << (D * (l - 1));
-        for (unsigned p0 = 0; p0 < npar; p0 += kTileParents)
+        int tb = 64;
+        for (int cand : {256, 128}) {
+            if ((long)(npar / cand) * (1 << D) >= 512 && m2l_pipe_supported(D, KP, mode, cand)) {
+                tb = cand;
+                break;
+            }
+        }
+        const bool piped = m2l_pipe_supported(D, KP, mode, tb);
+        if (groups.empty() || groups.back().tb != tb || groups.back().pipelined != piped)
+            groups.push_back(M2LGroup{tb, (int)t.size(), 0, piped});
+        for (unsigned p0 = 0; p0 < npar; p0 += tb)
             for (int c = 0; c < (1 << D); ++c) {
-                const unsigned np = npar - p0 < (unsigned)kTileParents ? npar - p0 : (unsigned)kTileParents;
+                const unsigned np = npar - p0 < (unsigned)tb ? npar - p0 : (unsigned)tb;
                 t.push_back(M2LTask{l, c, p0, (int)np});
+                ++groups.back().count;
             }
     }
     return t;
@@ -348,29 +365,39 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
     }
 }
 
-// Software-pipelined contraction for a compile-time padded length KP = 4*KQ.
-// Thread t gathers column t/4 (one target box's partner), elements
-// [(t%4)*KQ, +KQ), into registers for offset i+1 while the MFMAs of offset i
-// read LDS; the operator fragments of offset i+1 are prefetched the same way.
-template <typename TW, int D, int MODE, int KQ>
+// Software-pipelined contraction, compile-time padded length KP and tile
+// width TB (target boxes per workgroup, 64..256).  Thread t gathers TPC = 256/TB
+// threads' share of one target's partner: column t / TPC, elements
+// [(t % TPC) * EPT, +EPT) of the padded operand copy, with 16-byte loads, for
+// offset i+1 while the MFMAs of offset i read LDS; the operator fragments of
+// offset i+1 are prefetched the same way.  Accumulators stay in registers
+// across all offsets: FP64 in MODE 0, FP32 (the MFMA accumulator) in MODES 1-2.
+template <typename TW, int D, int MODE, int KP, int TB>
 __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 {
-    constexpr int KP = 4 * KQ;
     constexpr int NKS = MODE == 2 ? KP / 32 : KP / 4;
+    constexpr int TPC = 256 / TB, EPT = KP / TPC, NCB = TB / 16;
     using CT = typename std::conditional<MODE == 0, double, float>::type;
     using V4 = typename std::conditional<MODE == 0, d4_t, f4_t>::type;
     using ST = typename std::conditional<MODE == 2, __bf16, CT>::type;
     using AT = typename std::conditional<MODE == 2, bf8_t, CT>::type;
     constexpr int LDB = MODE == 2 ? KP + 8 : KP + 1;
     constexpr int NPART = MODE == 2 ? 2 : 1;
-    __shared__ __align__(16) ST Bs[NPART * 64 * LDB];
+    constexpr int VEC = 16 / (int)sizeof(ST);  // elements per 16-byte load
+    static_assert(EPT % VEC == 0, "gather chunks must be whole 16-byte vectors");
+    __shared__ __align__(16) ST Bs[NPART * TB * LDB];
 
-    const M2LTask task = a.tasks[blockIdx.x];
+    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
+    // contiguous run of tasks (neighbouring tiles, all classes) so that its
+    // L2 holds the multipoles and operators those tasks share.
+    const unsigned nt8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
+    const unsigned tix = xcd * nt8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const M2LTask task = a.tasks[tix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int rb = blockIdx.y * 4 + wave;
     const bool active = rb * 16 < a.MP;
     const int N = 1 << task.level, P = a.P, nrb = a.MP / 16;
-    const int col = tid >> 2, part = tid & 3;
+    const int col = tid / TPC, part = tid % TPC;
 
     int tc[D];
     const bool cvalid = col < task.np;
@@ -380,11 +407,13 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
         for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((task.cls >> (D - 1 - d)) & 1);
     }
-    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
+    const ST *Wg = static_cast<const ST *>(a.Wc) + a.lvlc_off[task.level];
+    [[maybe_unused]] const ST *Wg_lo = static_cast<const ST *>(a.Wc_lo) + a.lvlc_off[task.level];
     const int *cls_f = a.cls_f + task.cls * a.n_cls;
     const int *cls_o = a.cls_o + task.cls * a.n_cls * D;
 
-    TW g[KQ];
+    uint4 g[EPT / VEC];
+    [[maybe_unused]] uint4 g_lo[MODE == 2 ? EPT / VEC : 1];
     auto gather = [&](int i) {
         int m[D];
         bool ok = cvalid;
@@ -393,95 +422,85 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
             m[d] = tc[d] + cls_o[i * D + d];
             ok = ok && m[d] >= 0 && m[d] < N;
         }
-        const TW *src = Wm + (size_t)(ok ? morton_encode<D>(m) : 0u) * P;
+        const size_t off = (size_t)(ok ? morton_encode<D>(m) : 0u) * KP + part * EPT;
+        const uint4 *src = reinterpret_cast<const uint4 *>(Wg + off);
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) {
-            const int k = part * KQ + j;
-            g[j] = (ok && k < P) ? src[k] : TW(0);
+        for (int j = 0; j < EPT / VEC; ++j) g[j] = ok ? src[j] : uint4{0, 0, 0, 0};
+        if constexpr (MODE == 2) {
+            const uint4 *sl = reinterpret_cast<const uint4 *>(Wg_lo + off);
+#pragma unroll
+            for (int j = 0; j < EPT / VEC; ++j) g_lo[j] = ok ? sl[j] : uint4{0, 0, 0, 0};
         }
     };
     AT afr[NKS];
+    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKS : 1];
     auto load_a = [&](int i) {
         if (!active) return;
         const size_t f = (size_t)cls_f[i];
+        const size_t base = (f * nrb + rb) * NKS * 64;
         if constexpr (MODE == 2) {
             const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + a.kf_off[task.level]);
-            const size_t base = (f * nrb + rb) * NKS * 64;
+            const bf8_t *Kl =
+                reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) afr[s] = Kh[base + s * 64 + lane];
+            for (int s = 0; s < NKS; ++s) {
+                afr[s] = Kh[base + s * 64 + lane];
+                afr_lo[s] = Kl[base + s * 64 + lane];
+            }
         } else {
             const CT *Kf = static_cast<const CT *>(a.Kf) + a.kf_off[task.level];
-            const size_t base = (f * nrb + rb) * NKS * 64;
 #pragma unroll
             for (int s = 0; s < NKS; ++s) afr[s] = Kf[base + s * 64 + lane];
         }
     };
-    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKS : 1];
-    auto load_a_lo = [&](int i) {
-        if constexpr (MODE == 2) {
-            if (!active) return;
-            const size_t f = (size_t)cls_f[i];
-            const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
-            const size_t base = (f * nrb + rb) * NKS * 64;
-#pragma unroll
-            for (int s = 0; s < NKS; ++s) afr_lo[s] = Kl[base + s * 64 + lane];
-        }
-    };
 
-    TW acc[4][4];
+    V4 acc[NCB];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[c][q] = TW(0);
+    for (int cb = 0; cb < NCB; ++cb) acc[cb] = V4{0, 0, 0, 0};
 
     const int n = a.n_cls;
     gather(0);
     load_a(0);
-    load_a_lo(0);
     for (int i = 0; i < n; ++i) {
-        // registers -> LDS tile of offset i
-#pragma unroll
-        for (int j = 0; j < KQ; ++j) {
-            const int k = part * KQ + j;
+        {   // registers -> LDS tile of offset i
+            ST *dst = Bs + col * LDB + part * EPT;
             if constexpr (MODE == 2) {
-                const float v = (float)g[j];
-                const __bf16 h = (__bf16)v;
-                Bs[col * LDB + k] = h;
-                Bs[64 * LDB + col * LDB + k] = (__bf16)(v - (float)h);
+#pragma unroll
+                for (int j = 0; j < EPT / VEC; ++j) {
+                    reinterpret_cast<uint4 *>(dst)[j] = g[j];
+                    reinterpret_cast<uint4 *>(dst + TB * LDB)[j] = g_lo[j];
+                }
             } else {
-                Bs[col * LDB + k] = (CT)g[j];
+                const ST *v = reinterpret_cast<const ST *>(g);
+#pragma unroll
+                for (int j = 0; j < EPT; ++j) dst[j] = v[j];
             }
         }
         __syncthreads();
         AT acur[NKS];
-#pragma unroll
-        for (int s = 0; s < NKS; ++s) acur[s] = afr[s];
         [[maybe_unused]] AT acur_lo[MODE == 2 ? NKS : 1];
-        if constexpr (MODE == 2) {
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) acur_lo[s] = afr_lo[s];
+        for (int s = 0; s < NKS; ++s) {
+            acur[s] = afr[s];
+            if constexpr (MODE == 2) acur_lo[s] = afr_lo[s];
         }
         if (i + 1 < n) {
             gather(i + 1);
             load_a(i + 1);
-            load_a_lo(i + 1);
         }
         if (active) {
-            V4 c[4];
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) c[cb] = V4{0, 0, 0, 0};
             if constexpr (MODE == 2) {
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) {
                     const int kk = 32 * s + 8 * (lane >> 4);
 #pragma unroll
-                    for (int cb = 0; cb < 4; ++cb) {
+                    for (int cb = 0; cb < NCB; ++cb) {
                         const int off = (16 * cb + (lane & 15)) * LDB + kk;
                         const bf8_t bh = *reinterpret_cast<const bf8_t *>(Bs + off);
-                        const bf8_t bl = *reinterpret_cast<const bf8_t *>(Bs + 64 * LDB + off);
-                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[s], bh, c[cb], 0, 0, 0);
-                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bl, c[cb], 0, 0, 0);
-                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bh, c[cb], 0, 0, 0);
+                        const bf8_t bl = *reinterpret_cast<const bf8_t *>(Bs + TB * LDB + off);
+                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[s], bh, acc[cb], 0, 0, 0);
+                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bl, acc[cb], 0, 0, 0);
+                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bh, acc[cb], 0, 0, 0);
                     }
                 }
             } else {
@@ -489,104 +508,161 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                 for (int s = 0; s < NKS; ++s) {
                     const int kk = 4 * s + (lane >> 4);
 #pragma unroll
-                    for (int cb = 0; cb < 4; ++cb) {
+                    for (int cb = 0; cb < NCB; ++cb) {
                         const CT bv = Bs[(16 * cb + (lane & 15)) * LDB + kk];
                         if constexpr (MODE == 0)
-                            c[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[s], bv, c[cb], 0, 0, 0);
+                            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[s], bv, acc[cb], 0, 0, 0);
                         else
-                            c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[s], bv, c[cb], 0, 0, 0);
+                            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[s], bv, acc[cb], 0, 0, 0);
                     }
                 }
             }
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[cb][q] += (TW)c[cb][q];
         }
         __syncthreads();
     }
     if (!active) return;
     TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int cb = 0; cb < NCB; ++cb) {
         const int cc = 16 * cb + (lane & 15);
         if (cc >= task.np) continue;
         const size_t box = ((size_t)(task.p0 + cc) << D) | (unsigned)task.cls;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
-            if (row < P) Wl[box * P + row] = acc[cb][q];
+            if (row < P) Wl[box * P + row] = (TW)acc[cb][q];
         }
     }
 }
 
-template <typename T, int D, int MODE, int KQ>
-static bool launch_pipe_kq(const M2LKArgs &a, dim3 grid, hipStream_t st)
+// Tile widths with a pipelined kernel for (MODE, KP, TB); the others use
+// m2l_mfma4 / m2l_bf16x3.
+template <typename T, int D, int MODE, int KP, int TB>
+static bool launch_pipe_tb(const M2LKArgs &a, dim3 grid, hipStream_t st)
 {
-    if constexpr (MODE == 2 && KQ % 8 != 0) {
-        return false;  // split-bf16 fragments need KP a multiple of 32
+    constexpr int VEC = MODE == 2 ? 8 : MODE == 1 ? 4 : 2;
+    if constexpr ((KP / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128)) {
+        return false;
     } else {
-        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KQ>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB>), grid, dim3(256), 0, st, a);
         return true;
     }
 }
 
-template <typename T, int D, int MODE>
-static bool launch_pipe(const M2LKArgs &a, dim3 grid, hipStream_t st)
+template <typename T, int D, int MODE, int KP>
+static bool launch_pipe_kp(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 {
-    switch (a.KP / 4) {
-    case 2: return launch_pipe_kq<T, D, MODE, 2>(a, grid, st);
-    case 4: return launch_pipe_kq<T, D, MODE, 4>(a, grid, st);
-    case 7: return launch_pipe_kq<T, D, MODE, 7>(a, grid, st);
-    case 8: return launch_pipe_kq<T, D, MODE, 8>(a, grid, st);
-    case 9: return launch_pipe_kq<T, D, MODE, 9>(a, grid, st);
-    case 16: return launch_pipe_kq<T, D, MODE, 16>(a, grid, st);
-    case 32: return launch_pipe_kq<T, D, MODE, 32>(a, grid, st);
+    if (tb == 64) return launch_pipe_tb<T, D, MODE, KP, 64>(a, grid, st);
+    if constexpr (D == 3) {
+        if (tb == 128) return launch_pipe_tb<T, D, MODE, KP, 128>(a, grid, st);
+        if (tb == 256) return launch_pipe_tb<T, D, MODE, KP, 256>(a, grid, st);
     }
     return false;
 }
 
-template <typename T, int D>
-int m2l_apply(const M2LKArgs &a, int n_tasks, int mode, hipStream_t st)
+template <typename T, int D, int MODE>
+static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 {
-    if (n_tasks <= 0) return FMMA_OK;
-    const dim3 grid((unsigned)n_tasks, (unsigned)((a.MP + 63) / 64));
-    bool done = false;
-    if (mode == 0) {
-        if constexpr (std::is_same<T, double>::value) {
-            done = launch_pipe<T, D, 0>(a, grid, st);
+    switch (a.KP) {
+    case 32: return launch_pipe_kp<T, D, MODE, 32>(a, tb, grid, st);
+    case 36: return launch_pipe_kp<T, D, MODE, 36>(a, tb, grid, st);
+    case 64: return launch_pipe_kp<T, D, MODE, 64>(a, tb, grid, st);
+    case 128: return launch_pipe_kp<T, D, MODE, 128>(a, tb, grid, st);
+    }
+    return false;
+}
+
+bool m2l_pipe_supported(int D, int KP, int mode, int tb)
+{
+    const int vec = mode == 2 ? 8 : mode == 1 ? 4 : 2;
+    const bool kp_ok = KP == 32 || KP == 36 || KP == 64 || KP == 128;
+    if (!kp_ok || (mode == 2 && KP % 32) || (KP / (256 / tb)) % vec) return false;
+    if (tb == 64) return true;
+    if (D != 3) return false;
+    return mode == 0 ? tb == 128 : (tb == 128 || tb == 256);
+}
+
+template <typename T, int D>
+int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode, hipStream_t st)
+{
+    for (const M2LGroup &g : groups) {
+        if (g.count <= 0) continue;
+        M2LKArgs a = a0;
+        a.tasks = a0.tasks + g.first;
+        const dim3 grid((unsigned)g.count, (unsigned)((a.MP + 63) / 64));
+        bool done = false;
+        if (mode == 0) {
+            if constexpr (std::is_same<T, double>::value) {
+                done = g.pipelined && launch_pipe<T, D, 0>(a, g.tb, grid, st);
+                if (!done) {
+                    const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
+                    hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
+                }
+            } else {
+                return FMMA_ERR_INVALID;
+            }
+        } else if (mode == 1) {
+            done = g.pipelined && launch_pipe<T, D, 1>(a, g.tb, grid, st);
             if (!done) {
-                const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
-                hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
+                const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
+                hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
             }
         } else {
-            return FMMA_ERR_INVALID;
+            done = g.pipelined && launch_pipe<T, D, 2>(a, g.tb, grid, st);
+            if (!done) {
+                const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);
+                hipLaunchKernelGGL((m2l_bf16x3<T, D>), grid, dim3(256), lds, st, a);
+            }
         }
-    } else if (mode == 1) {
-        done = launch_pipe<T, D, 1>(a, grid, st);
-        if (!done) {
-            const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
-            hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
-        }
-    } else {
-        done = launch_pipe<T, D, 2>(a, grid, st);
-        if (!done) {
-            const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);
-            hipLaunchKernelGGL((m2l_bf16x3<T, D>), grid, dim3(256), lds, st, a);
-        }
+        FMMA_LAUNCHED("m2l_apply");
     }
-    FMMA_LAUNCHED("m2l_apply");
     return FMMA_OK;
 }
 
+// Multipoles of levels 2..L as MFMA operands, one row of KP (zero padded)
+// per box: TYPE (mode 0), FP32 (mode 1), or split BF16 hi = bf16(v),
+// lo = bf16(v - hi) of v = (float)W (mode 2).
+template <typename T>
+__global__ __launch_bounds__(256) void convert_operands(const T *W, long nbox, int P, int KP, int mode, void *out,
+                                                        __bf16 *lo)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nbox * KP) return;
+    const long b = e / KP;
+    const int k = (int)(e - b * KP);
+    const T w = k < P ? W[b * P + k] : T(0);
+    if (mode == 0) {
+        static_cast<T *>(out)[e] = w;
+    } else if (mode == 1) {
+        static_cast<float *>(out)[e] = (float)w;
+    } else {
+        const float v = (float)w;
+        const __bf16 h = (__bf16)v;
+        static_cast<__bf16 *>(out)[e] = h;
+        lo[e] = (__bf16)(v - (float)h);
+    }
+}
+
+template <typename T>
+int m2l_convert(const T *W, long nbox, int P, int KP, int mode, void *Wc, void *Wc_lo, hipStream_t st)
+{
+    if (nbox <= 0) return FMMA_OK;
+    const unsigned nb = (unsigned)((nbox * KP + 255) / 256);
+    hipLaunchKernelGGL((convert_operands<T>), dim3(nb), dim3(256), 0, st, W, nbox, P, KP, mode, Wc, (__bf16 *)Wc_lo);
+    FMMA_LAUNCHED("m2l_convert");
+    return FMMA_OK;
+}
+
+template int m2l_convert<double>(const double *, long, int, int, int, void *, void *, hipStream_t);
+template int m2l_convert<float>(const float *, long, int, int, int, void *, void *, hipStream_t);
 template int m2l_fragments<double>(const double *, int, int, int, void *, void *, hipStream_t);
 template int m2l_fragments<float>(const float *, int, int, int, void *, void *, hipStream_t);
-template int m2l_apply<double, 1>(const M2LKArgs &, int, int, hipStream_t);
-template int m2l_apply<double, 2>(const M2LKArgs &, int, int, hipStream_t);
-template int m2l_apply<double, 3>(const M2LKArgs &, int, int, hipStream_t);
-template int m2l_apply<float, 1>(const M2LKArgs &, int, int, hipStream_t);
-template int m2l_apply<float, 2>(const M2LKArgs &, int, int, hipStream_t);
-template int m2l_apply<float, 3>(const M2LKArgs &, int, int, hipStream_t);
+template int m2l_apply<double, 1>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
+template int m2l_apply<double, 2>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
+template int m2l_apply<double, 3>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
+template int m2l_apply<float, 1>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
+template int m2l_apply<float, 2>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
+template int m2l_apply<float, 3>(const M2LKArgs &, const std::vector<M2LGroup> &, int, hipStream_t);
 
 }  // namespace lib
 }  // namespace fmma

@@ -30,7 +30,14 @@ struct M2LTask {
     int np;
 };
 
-std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level);
+// A run of tasks with one tile width (one launch).
+struct M2LGroup {
+    int tb, first, count;
+    bool pipelined;
+};
+
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups);
+bool m2l_pipe_supported(int D, int KP, int mode, int tb);
 
 // Kernel arguments of the MFMA contraction.
 struct M2LKArgs {
@@ -45,6 +52,9 @@ struct M2LKArgs {
     void *Wl;
     const void *Kf;             // operator fragments (f64 / f32), or bf16 hi part
     const void *Kf_lo;          // bf16 lo part
+    const void *Wc;             // multipoles as MFMA operands, rows of KP: TYPE / float / bf16 hi
+    const void *Wc_lo;          // bf16 lo part (mode 2)
+    long lvlc_off[kMaxLevels];  // element offset of level l in Wc / Wc_lo
 };
 
 // Shape of the operator fragments for a precision: rows MP, contraction KP.
@@ -55,9 +65,13 @@ size_t m2l_frag_elems(int n_far, int P, int precision);  // elements per level (
 template <typename T>
 int m2l_fragments(const T *Kraw, int n_far, int P, int precision, void *Kf, void *Kf_lo, hipStream_t st);
 
-// All levels at once: one workgroup per task, blockDim 256, grid.y = row chunks.
+// nbox multipole rows of P converted to padded operand rows of KP.
+template <typename T>
+int m2l_convert(const T *W, long nbox, int P, int KP, int mode, void *Wc, void *Wc_lo, hipStream_t st);
+
+// One launch per task group: workgroup per task, blockDim 256, grid.y = row chunks.
 template <typename T, int D>
-int m2l_apply(const M2LKArgs &a, int n_tasks, int precision, hipStream_t st);
+int m2l_apply(const M2LKArgs &a, const std::vector<M2LGroup> &groups, int mode, hipStream_t st);
 
 }  // namespace lib
 }  // namespace fmma

@@ -95,7 +95,7 @@ struct Plan {
     DBuf geom, bbox_part;
     DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
     DBuf t_keys, t_vals, t_keys_alt, t_vals_alt, t_hist, t_pos, t_start;
-    DBuf Wm, Wl, Kraw, Kf, Kf_lo, l2p_out, exact_part;
+    DBuf Wm, Wl, Kraw, Kf, Kf_lo, Wc, Wc_lo, l2p_out, exact_part;
     DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks;
     DBuf h_t, h_s, h_w, h_ans;
 
@@ -103,6 +103,8 @@ struct Plan {
     int set_n = -1, set_L = -1, set_mode = -1;
     OffsetTables offs;
     int n_tasks = 0;
+    std::vector<M2LGroup> groups;
+    long lvlc_off[kMaxLevels + 1] = {};
     long lvl_off[kMaxLevels + 1] = {};
     long kf_off[kMaxLevels] = {};
     int last_L = -1, last_shared = 0;
@@ -203,7 +205,18 @@ static int setup(Plan &pl, int L)
             if ((rc = pl.Kf.ensure(fe * esz * (L - 1)))) return rc;
             if (mode == 2 && (rc = pl.Kf_lo.ensure(fe * esz * (L - 1)))) return rc;
             if ((rc = pl.Kraw.ensure((size_t)pl.offs.n_far * P * P * pl.tsize()))) return rc;
-            std::vector<M2LTask> tasks = make_m2l_tasks(D, L);
+            // padded operand rows of KP for levels 2..L
+            int MP, KP;
+            m2l_shape(P, mode, MP, KP);
+            long offc = 0;
+            for (int l = 0; l <= L; ++l) {
+                pl.lvlc_off[l] = offc;
+                if (l >= 2) offc += (long)KP << (D * l);
+            }
+            const size_t opsz = mode == 0 ? pl.tsize() : mode == 1 ? 4 : 2;
+            if ((rc = pl.Wc.ensure(offc * opsz))) return rc;
+            if (mode == 2 && (rc = pl.Wc_lo.ensure(offc * 2))) return rc;
+            std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups);
             pl.n_tasks = (int)tasks.size();
             if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
             FMMA_HIP(hipStreamSynchronize(st));
@@ -313,7 +326,17 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
             ka.Wl = Wl;
             ka.Kf = pl.Kf.p;
             ka.Kf_lo = pl.Kf_lo.p;
-            if ((rc = m2l_apply<T, D>(ka, pl.n_tasks, mode, st))) return rc;
+            ka.Wc = pl.Wc.p;
+            ka.Wc_lo = pl.Wc_lo.p;
+            for (int l = 0; l < kMaxLevels; ++l) ka.lvlc_off[l] = l <= L ? pl.lvlc_off[l] : 0;
+            {
+                const size_t opsz = mode == 0 ? sizeof(T) : mode == 1 ? 4 : 2;
+                char *wc = pl.Wc.as<char>() + pl.lvlc_off[2] * opsz;
+                char *wl = mode == 2 ? pl.Wc_lo.as<char>() + pl.lvlc_off[2] * 2 : nullptr;
+                const long nbox = (pl.lvl_off[L + 1] - pl.lvl_off[2]) / P;
+                if ((rc = m2l_convert<T>(Wm + pl.lvl_off[2], nbox, P, ka.KP, mode, wc, wl, st))) return rc;
+            }
+            if ((rc = m2l_apply<T, D>(ka, pl.groups, mode, st))) return rc;
         } else {
             for (int l = 2; l <= L; ++l) {
                 fmma_m2l_general_args ga{D, pl.dtype, n, P, pl.d_nodes.as<double>(), l,
