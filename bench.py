@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--forms", default="fp32,native,bf16x3")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (3 times): the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -79,6 +82,20 @@ def pair_count(x_np, depth):
 
 def m2l_pairs(depth, dim=3):
     return sum((6 * 2 ** l - 8) ** dim - (3 * 2 ** l - 2) ** dim for l in range(2, depth))
+
+
+def measured_traffic(kernel, n, depth):
+    """HBM-side bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled
+    by the plain-copy calibration) from the committed rocprofv3 --pmc summary
+    of this workload, or None when the summary does not cover this run."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_3d_n4m_depth6.md")
+    if n != 1 << 22 or depth != 6 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.split("|")]
+        if len(cells) > 5 and kernel in cells[1]:
+            return int(cells[4]) + int(cells[5])
+    return None
 
 
 def cpu_baseline():
@@ -133,6 +150,14 @@ def main():
     from fmma_amd import synth
 
     dev = torch.device("cuda:0")
+    if args.copy_mib:
+        a = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}))
+        return
     n = args.n
     x_np = synth.points(1, n, 3)
     w_np = synth.uniform(2, n)
@@ -206,16 +231,16 @@ def main():
     if dom == "P2P":
         flops = pairs * P2P_FLOPS_PER_PAIR
         ach = flops / (stages["P2P"] * 1e-3)
-        roof = {"kernel": "p2p_kernel", "bound": "valu-fp64", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
+        roof = {"kernel": "p2p_kernel", "bound": "fp64-valu", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": measured_traffic("p2p_kernel", n, depth),
                 "work": "%d pair interactions x %d flops" % (pairs, P2P_FLOPS_PER_PAIR),
                 "pairs_per_s": pairs / (stages["P2P"] * 1e-3)}
     elif dom == "M2L":
         flops = 2 * P * P * mpairs
         ach = flops / (stages["M2L"] * 1e-3)
         peak = {"fp32": PEAK_FP32, "native": PEAK_FP64, "bf16x3": PEAK_BF16 / 3}[head]
-        roof = {"kernel": "m2l_mfma4", "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+        roof = {"kernel": "m2l_pipe", "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": measured_traffic("m2l_pipe", n, depth),
                 "work": "2*P^2 x %d M2L pairs" % mpairs}
     else:
         nbytes = n * 4 * 8 + (8 ** (depth - 1)) * P * 8

@@ -45,33 +45,34 @@ __device__ __forceinline__ T pair_eval(const K &k, const T *x, const T *y)
 
 // ----------------------------------------------------------------- P2P
 //
-// One workgroup of 256 threads per target leaf box (Morton order).  Thread t
-// is (slot s = t % 16, slice q = t / 16): slot s owns the box's targets
-// s, s+16, s+32, ... (up to kP2PTargets of them in registers per pass) and
-// slice q takes every 16th source of the neighbourhood.  The sources of the
-// 3^D clipped neighbour boxes (src/fmm_core.cpp:81-115), concatenated, are
-// staged through LDS in blocks of kP2PBlock as (x.., w) records, so every
-// source is read from memory once per box and from LDS by 16 threads at a
-// time.  Work is split evenly over all 256 threads whatever the box's target
-// count (granularity 16 targets), and each LDS read feeds up to 12 pair
-// evaluations.  The 16 slice partial sums of a target are added in a fixed
-// order (wave shuffles, then LDS), so results are deterministic.
-constexpr int kP2PTargets = 12;  // targets per thread per pass
-constexpr int kP2PBlock = 512;   // sources per LDS stage
+// One wave (64 threads) per target leaf box (Morton order).  Lane l is
+// (slot s = l % S, slice q = l / S) with S = 4..64 slots chosen per box
+// (the fewest that keep at most kP2PTargets targets per lane): slot s owns
+// the box's targets s, s+S, s+2S, ... in registers and slice q takes every
+// (64/S)-th source of the neighbourhood.  The sources of the 3^D clipped
+// neighbour boxes (src/fmm_core.cpp:81-115), the box itself first, are
+// staged through LDS in blocks of kP2PBlock (x.., w) records; every LDS read
+// feeds one pair evaluation per target of the lane.  The 64/S slice partial
+// sums of a target are added by xor shuffles in a fixed order, so results are
+// deterministic.  Records of the target's own box may coincide with a target
+// (f's r = 0 case); the other boxes use the functor's far() form if it has one.
+constexpr int kP2PTargets = 12;  // targets per lane per pass
+constexpr int kP2PBlock = 256;   // sources per LDS stage
 
 template <int D>
 struct p2p_rec {  // LDS record width in elements: coordinates + weight, padded to even
     static constexpr int W = (D + 2) & ~1;
 };
 
-// The pair loop over staged records [j0, j1) of slice q (indices = q mod 16)
-// for a compile-time number of targets.
+// The pair loop over staged records [j0, j1) of slice q (indices = q mod
+// nslice) for a compile-time number of targets.
 template <int TPL, bool FAR, typename T, int D, class K>
-__device__ __forceinline__ void p2p_range(const T *src, unsigned j0, unsigned j1, unsigned q,
+__device__ __forceinline__ void p2p_range(const T *src, unsigned j0, unsigned j1, unsigned q, unsigned nslice,
                                           const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
 {
     constexpr int RW = p2p_rec<D>::W;
-    for (unsigned j = j0 + ((q + 16u - (j0 & 15u)) & 15u); j < j1; j += 16) {
+    const unsigned r0 = j0 % nslice;
+    for (unsigned j = j0 + (q >= r0 ? q - r0 : q + nslice - r0); j < j1; j += nslice) {
         const T *r = src + j * RW;
         T y[D];
 #pragma unroll
@@ -82,77 +83,81 @@ __device__ __forceinline__ void p2p_range(const T *src, unsigned j0, unsigned j1
     }
 }
 
-// Records [0, nself) of the block come from the target's own box and may
-// coincide with a target; the rest cannot.
 template <int TPL, typename T, int D, class K>
-__device__ __forceinline__ void p2p_block(const T *src, unsigned cnt, unsigned nself, unsigned q,
+__device__ __forceinline__ void p2p_block(const T *src, unsigned cnt, unsigned nself, unsigned q, unsigned nslice,
                                           const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
 {
     const unsigned m = nself < cnt ? nself : cnt;
-    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, x, acc, kern);
-    p2p_range<TPL, true>(src, m, cnt, q, x, acc, kern);
+    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, nslice, x, acc, kern);
+    p2p_range<TPL, true>(src, m, cnt, q, nslice, x, acc, kern);
 }
 
 template <typename T, int D, class K>
-__global__ __launch_bounds__(256) void p2p_kernel(fmma_p2p_args a, K kern)
+__global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
 {
     constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
     constexpr int RW = p2p_rec<D>::W;
     __shared__ __align__(16) T src[kP2PBlock * RW];
-    __shared__ T red[4][16][kP2PTargets];
     __shared__ unsigned nb_first[NNB], nb_pre[NNB + 1];
 
     const unsigned box = blockIdx.x;
     const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
     if (t0 == t1) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int s = lane & 15;
-    const unsigned q = (unsigned)tid >> 4;
+    const unsigned lane = threadIdx.x;
     const T *tp = static_cast<const T *>(a.t_pos);
     const T *sp = static_cast<const T *>(a.s_pos);
     const T *sw = static_cast<const T *>(a.s_w);
     const long nt = a.n_t, ns = a.n_s;
 
     // neighbour ranges: the box itself first, then the other entries of the
-    // row-major 3^D block (last dimension fastest)
-    if (tid < NNB) {
+    // row-major 3^D block (last dimension fastest); prefix sum by a wave scan
+    unsigned cnt_nb = 0;
+    if (lane < (unsigned)NNB) {
         int bc[D], m[D];
         morton_decode<D>(box, bc);
         const int NL = 1 << a.leaf_level;
         bool ok = true;
         constexpr int centre = (NNB - 1) / 2;
-        int k = tid == 0 ? centre : tid <= centre ? tid - 1 : tid;
+        int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
 #pragma unroll
         for (int d = D - 1; d >= 0; --d) {
             m[d] = bc[d] + (k % 3) - 1;
             k /= 3;
             ok = ok && m[d] >= 0 && m[d] < NL;
         }
-        unsigned first = 0, cnt = 0;
+        unsigned first = 0;
         if (ok) {
             const unsigned mm = morton_encode<D>(m);
             first = a.s_start[mm];
-            cnt = a.s_start[mm + 1] - first;
+            cnt_nb = a.s_start[mm + 1] - first;
         }
-        nb_first[tid] = first;
-        nb_pre[tid + 1] = cnt;
+        nb_first[lane] = first;
     }
-    __syncthreads();
-    if (tid == 0) {
-        nb_pre[0] = 0;
-        for (int k = 0; k < NNB; ++k) nb_pre[k + 1] += nb_pre[k];
+    unsigned incl = cnt_nb;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += y;
     }
+    if (lane < (unsigned)NNB) nb_pre[lane + 1] = incl;
+    if (lane == 0) nb_pre[0] = 0;
     __syncthreads();
     const unsigned total = nb_pre[NNB], nself = nb_pre[1];
 
-    for (unsigned pass0 = t0; pass0 < t1; pass0 += 16 * kP2PTargets) {
-        const unsigned npass = t1 - pass0 < 16u * kP2PTargets ? t1 - pass0 : 16u * kP2PTargets;
-        const int tpl = (int)((npass + 15) / 16);
+    // slots: the fewest of 4, 8, .., 64 with <= kP2PTargets targets per lane
+    const unsigned ntb = t1 - t0;
+    unsigned nslot = 4;
+    while (nslot < 64 && nslot * kP2PTargets < ntb) nslot <<= 1;
+    const unsigned nslice = 64 / nslot, s = lane % nslot, q = lane / nslot;
+
+    for (unsigned pass0 = t0; pass0 < t1; pass0 += nslot * kP2PTargets) {
+        const unsigned npass = t1 - pass0 < nslot * kP2PTargets ? t1 - pass0 : nslot * kP2PTargets;
+        const int tpl = (int)((npass + nslot - 1) / nslot);
         T x[kP2PTargets][D], acc[kP2PTargets];
 #pragma unroll
         for (int k = 0; k < kP2PTargets; ++k) {
             acc[k] = T(0);
-            const unsigned i = pass0 + s + 16u * k;
+            const unsigned i = pass0 + s + nslot * k;
             const unsigned ic = i < t1 ? i : t1 - 1;
 #pragma unroll
             for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
@@ -160,9 +165,9 @@ __global__ __launch_bounds__(256) void p2p_kernel(fmma_p2p_args a, K kern)
         for (unsigned base = 0; base < total; base += kP2PBlock) {
             const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
             __syncthreads();  // previous block fully consumed
-            for (unsigned e = tid; e < cnt; e += 256) {
+            int b = 0;
+            for (unsigned e = lane; e < cnt; e += 64) {
                 const unsigned p = base + e;
-                int b = 0;
                 while (nb_pre[b + 1] <= p) ++b;
                 const long j = (long)nb_first[b] + (p - nb_pre[b]);
                 T *r = src + e * RW;
@@ -173,46 +178,44 @@ __global__ __launch_bounds__(256) void p2p_kernel(fmma_p2p_args a, K kern)
             __syncthreads();
             const unsigned ns_blk = nself > base ? nself - base : 0u;
             switch (tpl) {
-            case 1: p2p_block<1>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 2: p2p_block<2>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 3: p2p_block<3>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 4: p2p_block<4>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 5: p2p_block<5>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 6: p2p_block<6>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 7: p2p_block<7>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 8: p2p_block<8>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 9: p2p_block<9>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 10: p2p_block<10>(src, cnt, ns_blk, q, x, acc, kern); break;
-            case 11: p2p_block<11>(src, cnt, ns_blk, q, x, acc, kern); break;
-            default: p2p_block<12>(src, cnt, ns_blk, q, x, acc, kern); break;
+            case 1: p2p_block<1>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 2: p2p_block<2>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 3: p2p_block<3>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 4: p2p_block<4>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 5: p2p_block<5>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 6: p2p_block<6>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 7: p2p_block<7>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 8: p2p_block<8>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 9: p2p_block<9>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 10: p2p_block<10>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 11: p2p_block<11>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            default: p2p_block<12>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
             }
         }
-        // sum the 16 slices: q = 4*wave + lane/16
+        // sum the slices of each target: lanes s, s + nslot, s + 2 nslot, ...
 #pragma unroll
         for (int k = 0; k < kP2PTargets; ++k) {
             if (k < tpl) {
                 T v = acc[k];
-                v += __shfl_xor(v, 16);
-                v += __shfl_xor(v, 32);
-                if (lane < 16) red[wave][s][k] = v;
+                for (unsigned o = nslot; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
+                acc[k] = v;
             }
         }
-        __syncthreads();
-        for (int e = tid; e < 16 * tpl; e += 256) {
-            const int ss = e & 15, k = e >> 4;
-            const unsigned i = pass0 + ss + 16u * k;
-            if (i >= t1) continue;
-            const T v = red[0][ss][k] + red[1][ss][k] + red[2][ss][k] + red[3][ss][k];
-            T *ans = static_cast<T *>(a.ans);
-            const unsigned oi = a.t_perm[i];
-            if (a.base)
-                ans[oi] = static_cast<const T *>(a.base)[i] + v;
-            else if (a.accumulate)
-                ans[oi] += v;
-            else
-                ans[oi] = v;
+        if (q == 0) {
+#pragma unroll
+            for (int k = 0; k < kP2PTargets; ++k) {
+                const unsigned i = pass0 + s + nslot * k;
+                if (k >= tpl || i >= t1) continue;
+                T *ans = static_cast<T *>(a.ans);
+                const unsigned oi = a.t_perm[i];
+                if (a.base)
+                    ans[oi] = static_cast<const T *>(a.base)[i] + acc[k];
+                else if (a.accumulate)
+                    ans[oi] += acc[k];
+                else
+                    ans[oi] = acc[k];
+            }
         }
-        __syncthreads();  // red[] is reused by the next pass
     }
 }
 
@@ -220,7 +223,7 @@ template <typename T, int D, class K>
 int launch_p2p(const fmma_p2p_args *a, const K &k, hipStream_t s)
 {
     if (a->n_target_boxes <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(256), 0, s, *a, k);
+    hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }
 
