@@ -91,6 +91,11 @@ class Plan:
     def set_timing(self, on):
         _lib.check(self._L.fmma_plan_set_timing(self._h, 1 if on else 0))
 
+    def set_p2p_mutual(self, on):
+        """Allow the mutual P2P form (default): with targets = sources and a
+        symmetric kernel each pair of neighbour boxes is evaluated once."""
+        _lib.check(self._L.fmma_plan_set_p2p_mutual(self._h, 1 if on else 0))
+
     # ---- solve (src/fmma.cpp:143-207)
     def solve(self, targets, weights, sources, ans):
         """ans <- sum_j w_j f(x_i, y_j).  weights may be None (all ones).

@@ -18,7 +18,13 @@ LIB = os.path.join(HERE, "libfmma_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["tree.hip", "boxops.hip", "m2l.hip", "builtins.hip", "plan.hip"]
+# (source, object name, extra flags).  builtins.hip is compiled once per
+# (TYPE, DIM) so its kernel instantiations build in parallel, and once more
+# as the dispatcher over them.
+SOURCES = [("tree.hip", "tree", []), ("boxops.hip", "boxops", []), ("m2l.hip", "m2l", []),
+           ("plan.hip", "plan", []), ("builtins.hip", "builtins", [])] + [
+    ("builtins.hip", "builtins_%s%d" % (t[0], d), ["-DFMMA_BI_TYPE=" + t, "-DFMMA_BI_DIM=%d" % d])
+    for t in ("double", "float") for d in (1, 2, 3)]
 
 # Packed FP32 instructions are switched off for all device code.
 HIP_FLAGS = [
@@ -28,9 +34,10 @@ HIP_FLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(entry):
+    src, name, extra = entry
+    obj = os.path.join(OBJ, name + ".o")
+    cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed on %s:\n%s" % (src, r.stderr[-6000:]))
@@ -70,7 +77,7 @@ def build_cpp_tests():
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    jobs = min(len(SOURCES), max(1, (os.cpu_count() or 4)))
+    jobs = min(len(SOURCES), max(1, min(16, os.cpu_count() or 4)))
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"

@@ -1,4 +1,4 @@
-// boxops.hpp -- P2M / M2M / L2L / L2P entry points (boxops.hip).
+// boxops.hpp -- P2M / M2M / L2L / L2P and the P2P combine pass (boxops.hip).
 #pragma once
 #include "common.hpp"
 
@@ -32,6 +32,19 @@ struct L2PArgs {
     T *out;                 // [n] sorted order
 };
 
+// Final sums of the mutual P2P form (fmma_p2p_args::slots).
+template <typename T>
+struct P2PCombineArgs {
+    const unsigned *start;  // [nbox+1] point ranges
+    int leaf_level;
+    const T *slots;         // [(3^D-1)/2 + 1][n]
+    long n;
+    const T *base;          // [n] sorted order (L2P result), or NULL
+    const unsigned *perm;   // [n] original index of each sorted point
+    T *ans;                 // [n] original order
+    int accumulate;         // base == NULL: 1 adds onto ans, 0 overwrites
+};
+
 template <typename T, int D>
 int p2m(const P2MArgs<T> &a, long nbox, hipStream_t st);
 
@@ -45,6 +58,9 @@ int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream
 
 template <typename T, int D>
 int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);
+
+template <typename T, int D>
+int p2p_combine(const P2PCombineArgs<T> &a, long nbox, hipStream_t st);
 
 }  // namespace lib
 }  // namespace fmma

@@ -6,6 +6,10 @@
 // Newton step y = y0 (1 + e/2), e = 1 - r2 y0^2, which leaves at most 4.1e-15
 // (measured over 1e-8 <= r2 <= 1e2).
 //
+// `symmetry` is +1 when f(y,x) = f(x,y) and -1 when f(y,x) = -f(x,y), exactly
+// in floating point too (x - y and y - x differ only in sign); P2P uses it
+// to evaluate each pair of distinct neighbour boxes once.
+//
 // Kernels that special-case r = 0 also provide far(): the same function
 // without that test, which P2P uses for the 3^D - 1 neighbour boxes other
 // than the target's own box (points in different boxes never coincide).
@@ -47,6 +51,7 @@ __device__ __forceinline__ T dist2(const T *x, const T *y)
 
 struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.hpp:55-63)
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -63,6 +68,7 @@ struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.h
 
 struct KInvRReg {  // 1/|x-y|, 0 at r = 0
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -80,6 +86,7 @@ struct KInvRReg {  // 1/|x-y|, 0 at r = 0
 
 struct KInvDiff {  // 1/(x_0 - y_0)
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = -1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -90,6 +97,7 @@ struct KInvDiff {  // 1/(x_0 - y_0)
 
 struct KLogR {  // log|x-y| = 0.5 log r^2, 0 at r = 0
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -106,6 +114,7 @@ struct KLogR {  // log|x-y| = 0.5 log r^2, 0 at r = 0
 
 struct KGauss {  // exp(-r^2 / (2 h^2)); p1 = 1 / (2 h^2)
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -117,6 +126,7 @@ struct KGauss {  // exp(-r^2 / (2 h^2)); p1 = 1 / (2 h^2)
 
 struct KYukawa {  // exp(-kappa r) / r, 0 at r = 0; p0 = kappa
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -137,6 +147,7 @@ struct KYukawa {  // exp(-kappa r) / r, 0 at r = 0; p0 = kappa
 
 struct KSumR2 {  // sum_d (x_d - y_d) / r^2
     static constexpr bool translation_invariant = true;
+    static constexpr int symmetry = -1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -154,6 +165,7 @@ struct KSumR2 {  // sum_d (x_d - y_d) / r^2
 
 struct KSum2R2 {  // sum_d (x_d + y_d)^2 / r^2: depends on x and y, not only x - y
     static constexpr bool translation_invariant = false;
+    static constexpr int symmetry = 1;
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const

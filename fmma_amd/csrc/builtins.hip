@@ -37,6 +37,7 @@ struct builtin_ops {
         o->dim = D;
         o->dtype = sizeof(T) == 8 ? FMMA_F64 : FMMA_F32;
         o->translation_invariant = K::translation_invariant ? 1 : 0;
+        o->symmetry = detail::kernel_symmetry<K>::value;
         o->ctx = ctx;
         o->p2p = &p2p;
         o->exact = &exact;
@@ -45,8 +46,11 @@ struct builtin_ops {
     }
 };
 
+// Compiled once per (TYPE, DIM) with FMMA_BI_TYPE and FMMA_BI_DIM defined, so
+// that the eight kernels' instantiations build in parallel; compiled without
+// them it is the dispatcher over those six objects.
 template <typename T, int D>
-static int fill_kernel(int id, void *ctx, fmma_kernel_ops *o)
+int fill_builtin(int id, void *ctx, fmma_kernel_ops *o)
 {
     switch (id) {
     case FMMA_KERNEL_INV_R: builtin_ops<T, D, KInvR>::fill(o, ctx); return FMMA_OK;
@@ -61,19 +65,30 @@ static int fill_kernel(int id, void *ctx, fmma_kernel_ops *o)
     return FMMA_ERR_INVALID;
 }
 
+#if defined(FMMA_BI_TYPE) && defined(FMMA_BI_DIM)
+template int fill_builtin<FMMA_BI_TYPE, FMMA_BI_DIM>(int, void *, fmma_kernel_ops *);
+#else
+extern template int fill_builtin<double, 1>(int, void *, fmma_kernel_ops *);
+extern template int fill_builtin<double, 2>(int, void *, fmma_kernel_ops *);
+extern template int fill_builtin<double, 3>(int, void *, fmma_kernel_ops *);
+extern template int fill_builtin<float, 1>(int, void *, fmma_kernel_ops *);
+extern template int fill_builtin<float, 2>(int, void *, fmma_kernel_ops *);
+extern template int fill_builtin<float, 3>(int, void *, fmma_kernel_ops *);
+
 int builtin_kernel_ops(int id, int dim, int dtype, void *ctx, fmma_kernel_ops *out)
 {
     if (dtype == FMMA_F64) {
-        if (dim == 1) return fill_kernel<double, 1>(id, ctx, out);
-        if (dim == 2) return fill_kernel<double, 2>(id, ctx, out);
-        if (dim == 3) return fill_kernel<double, 3>(id, ctx, out);
+        if (dim == 1) return fill_builtin<double, 1>(id, ctx, out);
+        if (dim == 2) return fill_builtin<double, 2>(id, ctx, out);
+        if (dim == 3) return fill_builtin<double, 3>(id, ctx, out);
     } else {
-        if (dim == 1) return fill_kernel<float, 1>(id, ctx, out);
-        if (dim == 2) return fill_kernel<float, 2>(id, ctx, out);
-        if (dim == 3) return fill_kernel<float, 3>(id, ctx, out);
+        if (dim == 1) return fill_builtin<float, 1>(id, ctx, out);
+        if (dim == 2) return fill_builtin<float, 2>(id, ctx, out);
+        if (dim == 3) return fill_builtin<float, 3>(id, ctx, out);
     }
     return FMMA_ERR_INVALID;
 }
+#endif
 
 }  // namespace lib
 }  // namespace fmma

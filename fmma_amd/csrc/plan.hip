@@ -74,8 +74,9 @@ static double host_S(int n, double x, double y)
     return s;
 }
 
-static const char *kStageNames[] = {"tree", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P"};
-constexpr int kStages = 7;
+// "P2P sum" is the combine pass of the mutual P2P form (zero for the direct form)
+static const char *kStageNames[] = {"tree", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "P2P sum"};
+constexpr int kStages = 8;
 
 struct Plan {
     int dim = 3, dtype = FMMA_F64;
@@ -87,6 +88,7 @@ struct Plan {
     int precision = FMMA_M2L_NATIVE;
     hipStream_t stream = nullptr;
     bool timing = true;
+    bool p2p_mutual = true;
     std::map<std::string, double> time_log;
     std::vector<std::string> names_out;
     hipEvent_t ev[kStages + 1] = {};
@@ -95,7 +97,7 @@ struct Plan {
     DBuf geom, bbox_part;
     DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
     DBuf t_keys, t_vals, t_keys_alt, t_vals_alt, t_hist, t_pos, t_start;
-    DBuf Wm, Wl, Kraw, Kf, Kf_lo, Wc, Wc_lo, l2p_out, exact_part;
+    DBuf Wm, Wl, Kraw, Kf, Kf_lo, Wc, Wc_lo, l2p_out, exact_part, p2p_slots;
     DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks;
     DBuf h_t, h_s, h_w, h_ans;
 
@@ -363,10 +365,20 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
     }
     pl.mark(6);
 
+    // mutual form: the same points on both sides and a kernel that declares its symmetry
+    const bool mutual = shared && pl.p2p_mutual && L >= 1 && (pl.ops.symmetry == 1 || pl.ops.symmetry == -1);
+    const int n_upper = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
+    if (mutual && (rc = pl.p2p_slots.ensure((size_t)(n_upper + 1) * nt * sizeof(T)))) return rc;
     fmma_p2p_args qa{D, pl.dtype, (long)nbox, L, t_start, pl.s_start.as<unsigned>(), t_pos, pl.s_pos.as<T>(),
-                     pl.s_w.as<T>(), nt, ns, base, t_perm, ans, L < 2 ? 1 : 0};
+                     pl.s_w.as<T>(), nt, ns, base, t_perm, ans, L < 2 ? 1 : 0,
+                     mutual ? pl.p2p_slots.p : nullptr, mutual ? pl.ops.symmetry : 0};
     if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
     pl.mark(7);
+    if (mutual) {
+        P2PCombineArgs<T> ca{t_start, L, pl.p2p_slots.as<T>(), nt, base, t_perm, ans, L < 2 ? 1 : 0};
+        if ((rc = p2p_combine<T, D>(ca, nbox, st))) return rc;
+    }
+    pl.mark(8);
     pl.last_L = L;
     pl.last_shared = shared ? 1 : 0;
     pl.last_nt = nt;
@@ -436,11 +448,11 @@ static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long n
         else rc = solve_typed<float, 3>(pl, nt, tg, w, ns, sr, ans, is_fmm);
     }
     if (rc) return rc;
-    if (!is_fmm) pl.mark(7);
+    if (!is_fmm) pl.mark(kStages);
     FMMA_HIP(hipStreamSynchronize(pl.stream));
     if (pl.timing && pl.events_ready) {
         float total = 0.f;
-        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[7]));
+        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
         if (is_fmm) {
             for (int i = 0; i < kStages; ++i) {
                 float ms = 0.f;
@@ -598,6 +610,13 @@ int fmma_plan_set_timing(fmma_plan plan, int enable)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.timing = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.p2p_mutual = enable != 0;
     return FMMA_OK;
 }
 

@@ -34,6 +34,17 @@ struct has_far<K, T, D,
                                                                                   (const T *)nullptr))>>
     : std::true_type {};
 
+// A functor may declare `static constexpr int symmetry`: +1 when f(y,x) =
+// f(x,y), -1 when f(y,x) = -f(x,y).  Without it P2P does not assume either.
+template <class K, typename = void>
+struct kernel_symmetry {
+    static constexpr int value = 0;
+};
+template <class K>
+struct kernel_symmetry<K, std::void_t<decltype(K::symmetry)>> {
+    static constexpr int value = K::symmetry;
+};
+
 template <bool FAR, typename T, int D, class K>
 __device__ __forceinline__ T pair_eval(const K &k, const T *x, const T *y)
 {
@@ -66,9 +77,9 @@ struct p2p_rec {  // LDS record width in elements: coordinates + weight, padded 
 
 // The pair loop over staged records [j0, j1) of slice q (indices = q mod
 // nslice) for a compile-time number of targets.
-template <int TPL, bool FAR, typename T, int D, class K>
+template <int TPL, bool FAR, typename T, int D, int NT, class K>
 __device__ __forceinline__ void p2p_range(const T *src, unsigned j0, unsigned j1, unsigned q, unsigned nslice,
-                                          const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
+                                          const T (&x)[NT][D], T (&acc)[NT], const K &kern)
 {
     constexpr int RW = p2p_rec<D>::W;
     const unsigned r0 = j0 % nslice;
@@ -219,11 +230,239 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
     }
 }
 
+// ------------------------------------------------------ P2P, mutual form
+//
+// Targets are the sources (the same sorted points) and f(y,x) = sym f(x,y).
+// One wave per box A, laid out as in p2p_kernel with S = 4, 8 or 16 slots.
+// The wave evaluates A against itself as the direct form does, then against
+// each neighbour B that follows A in the row-major 3^D block (the upper half,
+// (3^D-1)/2 offsets).  A staged record j of B meets each target t of the lane
+// once: w_j f(x_t, y_j) goes to t's sum and w_t f(x_t, y_j) to a partial for
+// j, which the S lanes of the slice add up by DPP row moves in a fixed order.
+// B's sums go to slots[u][j] (u = the offset's index in the upper half), A's
+// own to slots[n_upper][i]; p2p_combine (boxops.hip) adds them per point.
+constexpr int kP2PMutualTargets = 16;  // targets per lane per pass
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of 2^lg lanes, lg in {2, 3, 4} (within one DPP row
+// of 16 lanes); every lane of a group ends with the same value.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int lg)
+{
+    v += dpp_move<0xB1>(v);                // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp_move<0x4E>(v);                // quad_perm [2,3,0,1]: lane ^ 2
+    if (lg >= 3) v += dpp_move<0x141>(v);  // row_half_mirror: lane i <-> 7 - i
+    if (lg >= 4) v += dpp_move<0x140>(v);  // row_mirror: lane i <-> 15 - i
+    return v;
+}
+
+template <int TPL, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_range_mutual(const T *src, T *rev, unsigned j0, unsigned j1, unsigned q,
+                                                 unsigned nslice, bool lead, int lg, const T (&x)[NT][D],
+                                                 const T (&wt)[NT], T (&acc)[NT], const K &kern)
+{
+    constexpr int RW = p2p_rec<D>::W;
+    const unsigned r0 = j0 % nslice;
+    for (unsigned j = j0 + (q >= r0 ? q - r0 : q + nslice - r0); j < j1; j += nslice) {
+        const T *r = src + j * RW;
+        T y[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) y[d] = r[d];
+        const T wj = r[D];
+        T part = T(0);
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            const T f = pair_eval<true, T, D>(kern, x[k], y);
+            acc[k] += wj * f;
+            part += wt[k] * f;
+        }
+        part = group_sum(part, lg);
+        if (lead) rev[j] = part;
+    }
+}
+
+template <int TPL, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_block_mutual(const T *src, T *rev, unsigned cnt, unsigned m, unsigned q,
+                                                 unsigned nslice, bool lead, int lg, const T (&x)[NT][D],
+                                                 const T (&wt)[NT], T (&acc)[NT], const K &kern)
+{
+    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, nslice, x, acc, kern);
+    p2p_range_mutual<TPL>(src, rev, m, cnt, q, nslice, lead, lg, x, wt, acc, kern);
+}
+
+template <typename T, int D, class K>
+__global__ __launch_bounds__(64) void p2p_mutual_kernel(fmma_p2p_args a, K kern)
+{
+    constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
+    constexpr int NU = (NNB - 1) / 2;
+    constexpr int RW = p2p_rec<D>::W;
+    constexpr int MT = kP2PMutualTargets;
+    __shared__ __align__(16) T src[kP2PBlock * RW];
+    __shared__ T rev[kP2PBlock];
+    __shared__ unsigned nb_first[NU + 1], nb_pre[NU + 2];
+
+    const unsigned box = blockIdx.x;
+    const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
+    if (t0 == t1) return;
+    const unsigned lane = threadIdx.x;
+    const T *pos = static_cast<const T *>(a.s_pos);
+    const T *pw = static_cast<const T *>(a.s_w);
+    T *slots = static_cast<T *>(a.slots);
+    const long n = a.n_t;
+    constexpr T sym = (T)kernel_symmetry<K>::value;
+
+    // entry 0: the box itself; entry 1 + u: entry NU + 1 + u of the row-major
+    // 3^D block (last dimension fastest), whose centre is entry NU
+    unsigned cnt_nb = 0;
+    if (lane <= (unsigned)NU) {
+        int bc[D], m[D];
+        morton_decode<D>(box, bc);
+        const int NL = 1 << a.leaf_level;
+        bool ok = true;
+        int k = lane == 0 ? NU : NU + (int)lane;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            m[d] = bc[d] + (k % 3) - 1;
+            k /= 3;
+            ok = ok && m[d] >= 0 && m[d] < NL;
+        }
+        unsigned first = 0;
+        if (ok) {
+            const unsigned mm = morton_encode<D>(m);
+            first = a.s_start[mm];
+            cnt_nb = a.s_start[mm + 1] - first;
+        }
+        nb_first[lane] = first;
+    }
+    unsigned incl = cnt_nb;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += y;
+    }
+    if (lane <= (unsigned)NU) nb_pre[lane + 1] = incl;
+    if (lane == 0) nb_pre[0] = 0;
+    __syncthreads();
+    const unsigned total = nb_pre[NU + 1], nself = nb_pre[1];
+
+    // slots: the fewest of 4, 8, 16 with <= MT targets per lane (more passes beyond)
+    const unsigned ntb = t1 - t0;
+    int lg = 2;
+    while (lg < 4 && (1u << lg) * MT < ntb) ++lg;
+    const unsigned nslot = 1u << lg, nslice = 64 / nslot, s = lane % nslot, q = lane / nslot;
+    const bool lead = s == 0;
+
+    for (unsigned pass0 = t0; pass0 < t1; pass0 += nslot * MT) {
+        const unsigned npass = t1 - pass0 < nslot * MT ? t1 - pass0 : nslot * MT;
+        const int tpl = (int)((npass + nslot - 1) / nslot);
+        T x[MT][D], wt[MT], acc[MT];
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            acc[k] = T(0);
+            const unsigned i = pass0 + s + nslot * k;
+            const unsigned ic = i < t1 ? i : t1 - 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? pos[d * n + ic] : T(0);
+            wt[k] = k < tpl && i < t1 ? pw[i] : T(0);  // padding lanes add nothing to B
+        }
+        for (unsigned base = 0; base < total; base += kP2PBlock) {
+            const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
+            __syncthreads();  // previous block fully consumed
+            int b = 0;
+            for (unsigned e = lane; e < cnt; e += 64) {
+                const unsigned p = base + e;
+                while (nb_pre[b + 1] <= p) ++b;
+                const long j = (long)nb_first[b] + (p - nb_pre[b]);
+                T *r = src + e * RW;
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = pos[d * n + j];
+                r[D] = pw[j];
+            }
+            __syncthreads();
+            const unsigned m = nself > base ? (nself - base < cnt ? nself - base : cnt) : 0u;
+            switch (tpl) {
+            case 1: p2p_block_mutual<1>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 2: p2p_block_mutual<2>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 3: p2p_block_mutual<3>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 4: p2p_block_mutual<4>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 5: p2p_block_mutual<5>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 6: p2p_block_mutual<6>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 7: p2p_block_mutual<7>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 8: p2p_block_mutual<8>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 9: p2p_block_mutual<9>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 10: p2p_block_mutual<10>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 11: p2p_block_mutual<11>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 12: p2p_block_mutual<12>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 13: p2p_block_mutual<13>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 14: p2p_block_mutual<14>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            case 15: p2p_block_mutual<15>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            default: p2p_block_mutual<16>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
+            }
+            __syncthreads();
+            // the neighbours' sums of this block (records m .. cnt): the first
+            // pass stores, later passes add (the same thread owns an entry in
+            // every pass)
+            b = 0;
+            for (unsigned e = lane; e < cnt; e += 64) {
+                if (e < m) continue;
+                const unsigned p = base + e;
+                while (nb_pre[b + 1] <= p) ++b;
+                const long j = (long)nb_first[b] + (p - nb_pre[b]);
+                T *dst = slots + (long)(b - 1) * n + j;
+                const T v = sym * rev[e];
+                *dst = pass0 == t0 ? v : *dst + v;
+            }
+        }
+        // the box's own sums: add the slices of each target (lanes s + nslot q')
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            if (k < tpl) {
+                T v = acc[k];
+                for (unsigned o = nslot; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
+                acc[k] = v;
+            }
+        }
+        if (q == 0) {
+#pragma unroll
+            for (int k = 0; k < MT; ++k) {
+                const unsigned i = pass0 + s + nslot * k;
+                if (k < tpl && i < t1) slots[(long)NU * n + i] = acc[k];
+            }
+        }
+    }
+}
+
 template <typename T, int D, class K>
 int launch_p2p(const fmma_p2p_args *a, const K &k, hipStream_t s)
 {
     if (a->n_target_boxes <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+    if (a->slots) {
+        if constexpr (kernel_symmetry<K>::value == 1 || kernel_symmetry<K>::value == -1) {
+            if (a->t_pos != a->s_pos || a->t_start != a->s_start || a->n_t != a->n_s ||
+                a->symmetry != kernel_symmetry<K>::value)
+                return FMMA_ERR_INVALID;
+            hipLaunchKernelGGL((p2p_mutual_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a,
+                               k);
+        } else {
+            return FMMA_ERR_INVALID;
+        }
+    } else {
+        hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+    }
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }
 
@@ -432,6 +671,7 @@ struct ops_for {
         o.dim = D;
         o.dtype = sizeof(T) == 8 ? FMMA_F64 : FMMA_F32;
         o.translation_invariant = K::translation_invariant ? 1 : 0;
+        o.symmetry = kernel_symmetry<K>::value;
         o.ctx = const_cast<K *>(ctx);
         o.p2p = &p2p;
         o.exact = &exact;

@@ -25,6 +25,9 @@ struct UserKernel {
     // f(x - y) is translation invariant by construction; f(x, y) is taken as
     // general unless the caller sets trans_sym_flag (src/fmm_core.cpp:383).
     static constexpr bool translation_invariant = !TWO_ARG;
+    // a functor type may declare `static constexpr int symmetry = 1` (f(y,x) =
+    // f(x,y)) or -1 (f(y,x) = -f(x,y)) to let P2P evaluate each pair once
+    static constexpr int symmetry = kernel_symmetry<F>::value;
 
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const

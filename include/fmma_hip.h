@@ -79,6 +79,16 @@ typedef struct {
     const unsigned *t_perm;    /* [n_t] original index of each sorted target */
     void *ans;                 /* [n_t] caller's output, original order */
     int accumulate;            /* base==NULL: 1 adds onto ans (leaf level < 2), 0 overwrites */
+    /* Mutual form, used when slots != NULL: targets and sources are the same
+     * sorted points (t_pos == s_pos, t_start == s_start) and f(y,x) =
+     * symmetry * f(x,y).  Box A evaluates itself and each neighbour B that
+     * follows it in the row-major 3^dim block once, adding to A's sums and,
+     * times symmetry, to B's.  The launcher writes slots[u][i] (u < n_upper =
+     * (3^dim-1)/2: what point i of box B received from box B - o_u, the u-th
+     * such offset) and slots[n_upper][i] (point i's own sum), sorted order,
+     * and leaves ans alone; fmma's combine pass adds them in a fixed order. */
+    void *slots;               /* [n_upper+1][n_t] TYPE, or NULL for the direct form */
+    int symmetry;              /* +1 or -1 in the mutual form */
 } fmma_p2p_args;
 
 typedef struct {
@@ -123,6 +133,8 @@ typedef struct {
 typedef struct {
     int dim, dtype;
     int translation_invariant; /* 1: f(x,y) = g(x-y); M2L may use per-offset operators */
+    int symmetry;              /* +1: f(y,x) = f(x,y), -1: f(y,x) = -f(x,y), 0: neither or unknown;
+                                  nonzero lets P2P use the mutual form when targets are the sources */
     void *ctx;
     int (*p2p)(const fmma_p2p_args *, void *ctx, void *stream);
     int (*exact)(const fmma_exact_args *, void *ctx, void *stream);
@@ -179,6 +191,13 @@ int fmma_plan_set_stream(fmma_plan plan, void *stream);
 
 /* New: record per-stage HIP-event times into the time log (default on). */
 int fmma_plan_set_timing(fmma_plan plan, int enable);
+
+/* New: allow the mutual P2P form (default 1).  It is used only when targets
+ * and sources are the same array and the kernel declares a symmetry; it
+ * evaluates each pair of distinct neighbour boxes once instead of twice and
+ * needs (3^dim+1)/2 * n TYPE values of scratch.  The reference evaluates every
+ * target box against its whole neighbourhood (src/fmm.cpp:122-141). */
+int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable);
 
 /* ------------------------------------------------------------------ solve */
 

@@ -42,6 +42,16 @@ static double rel(const std::vector<T> &a, const std::vector<T> &b)
     return std::sqrt(num / den);
 }
 
+// f(x - y) = d_0 exp(-50 |d|^2): odd in d = x - y and finite at d = 0.
+struct OddKernel {
+    static constexpr int symmetry = -1;
+    __host__ __device__ double operator()(const std::array<double, 3> &d) const
+    {
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        return d[0] * std::exp(-50.0 * r2);
+    }
+};
+
 static int fails = 0;
 static void check(const char *name, double v, double tol)
 {
@@ -126,6 +136,26 @@ int main()
         f.set_solver_type("exact");
         f.solve(t, s, e);
         check("float_1d_fmm_vs_exact", rel(a, e), 1e-3);
+    }
+    {  // an odd user kernel that declares symmetry -1: the mutual P2P form
+       // (targets = sources) against the direct form and the exact solver
+        const size_t n = 20000;
+        auto x = pts<double, 3>(n);
+        std::vector<double> w(n);
+        for (auto &v : w) v = 2 * uni() - 1;
+        fmma::FMMA<double, 3> f;
+        f.set_fn(OddKernel{});
+        f.set_solver_type("fmm");
+        f.set_poly_ord(4);
+        f.set_Depth(4);
+        std::vector<double> a, b, e;
+        f.solve(x, w, x, a);
+        fmma_plan_set_p2p_mutual(f.plan(), 0);
+        f.solve(x, w, x, b);
+        f.set_solver_type("exact");
+        f.solve(x, w, x, e);
+        check("odd_mutual_vs_direct_3d", rel(a, b), 1e-13);
+        check("odd_fmm_vs_exact_3d", rel(a, e), 1e-2);
     }
     {  // time log keys of the reference (src/fmm.cpp:57-141, src/fmma.cpp:189)
         fmma::FMMA<double, 2> f;
