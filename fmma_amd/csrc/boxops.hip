@@ -517,52 +517,26 @@ static bool l2p_fast(const L2PArgs<T> &a, hipStream_t st)
 
 // ------------------------------------------------------------ P2P combine
 //
-// Sum of the mutual P2P form's slots for every point of box B: its own sum,
-// then what each box B - o_u (u ascending) added, where o_u is entry
-// n_upper + 1 + u of the row-major 3^D block.  Box B - o_u wrote slot u only if
-// it lies in the grid and holds points; a ballot over the offsets gives that
-// mask once per box.  The order is fixed, so results are deterministic.
+// Final sums of the mutual P2P form: point i's own sum, then slot u = 0 ..
+// n_upper - 1 (what box B - o_u added, or zero), in that fixed order.  One
+// thread per sorted point; every read is coalesced.
 
 template <typename T, int D>
-__global__ __launch_bounds__(64) void p2p_combine_kernel(P2PCombineArgs<T> a)
+__global__ __launch_bounds__(256) void p2p_combine_kernel(P2PCombineArgs<T> a)
 {
     constexpr int NU = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
-    const unsigned box = blockIdx.x;
-    const unsigned t0 = a.start[box], t1 = a.start[box + 1];
-    if (t0 == t1) return;
-    const unsigned lane = threadIdx.x;
-    bool ok = false;
-    if (lane < (unsigned)NU) {
-        int bc[D], m[D];
-        morton_decode<D>(box, bc);
-        const int NL = 1 << a.leaf_level;
-        int k = NU + 1 + (int)lane;
-        ok = true;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    T v = a.slots[(long)NU * a.n + i];
 #pragma unroll
-        for (int d = D - 1; d >= 0; --d) {
-            m[d] = bc[d] - ((k % 3) - 1);
-            k /= 3;
-            ok = ok && m[d] >= 0 && m[d] < NL;
-        }
-        if (ok) {
-            const unsigned mm = morton_encode<D>(m);
-            ok = a.start[mm] < a.start[mm + 1];
-        }
-    }
-    const unsigned long long mask = __ballot(ok);
-    for (unsigned i = t0 + lane; i < t1; i += 64) {
-        T v = a.slots[(long)NU * a.n + i];
-#pragma unroll
-        for (int u = 0; u < NU; ++u)
-            if ((mask >> u) & 1ull) v += a.slots[(long)u * a.n + i];
-        const unsigned oi = a.perm[i];
-        if (a.base)
-            a.ans[oi] = a.base[i] + v;
-        else if (a.accumulate)
-            a.ans[oi] += v;
-        else
-            a.ans[oi] = v;
-    }
+    for (int u = 0; u < NU; ++u) v += a.slots[(long)u * a.n + i];
+    const unsigned oi = a.perm[i];
+    if (a.base)
+        a.ans[oi] = a.base[i] + v;
+    else if (a.accumulate)
+        a.ans[oi] += v;
+    else
+        a.ans[oi] = v;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -619,16 +593,16 @@ int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st)
 }
 
 template <typename T, int D>
-int p2p_combine(const P2PCombineArgs<T> &a, long nbox, hipStream_t st)
+int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st)
 {
-    if (nbox <= 0 || a.n <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((p2p_combine_kernel<T, D>), dim3((unsigned)nbox), dim3(64), 0, st, a);
+    if (a.n <= 0) return FMMA_OK;
+    hipLaunchKernelGGL((p2p_combine_kernel<T, D>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
     FMMA_LAUNCHED("p2p_combine");
     return FMMA_OK;
 }
 
 #define FMMA_BOX_INST(T, D)                                                                 \
-    template int p2p_combine<T, D>(const P2PCombineArgs<T> &, long, hipStream_t);           \
+    template int p2p_combine<T, D>(const P2PCombineArgs<T> &, hipStream_t);                 \
     template int p2m<T, D>(const P2MArgs<T> &, long, hipStream_t);                          \
     template int m2m<T, D>(const T *, T *, long, const T *, int, int, hipStream_t);         \
     template int l2l<T, D>(const T *, T *, long, const T *, int, int, hipStream_t);         \

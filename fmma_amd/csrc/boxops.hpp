@@ -35,9 +35,7 @@ struct L2PArgs {
 // Final sums of the mutual P2P form (fmma_p2p_args::slots).
 template <typename T>
 struct P2PCombineArgs {
-    const unsigned *start;  // [nbox+1] point ranges
-    int leaf_level;
-    const T *slots;         // [(3^D-1)/2 + 1][n]
+    const T *slots;         // [(3^D-1)/2 + 1][n], every entry written by the P2P pass
     long n;
     const T *base;          // [n] sorted order (L2P result), or NULL
     const unsigned *perm;   // [n] original index of each sorted point
@@ -60,7 +58,7 @@ template <typename T, int D>
 int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);
 
 template <typename T, int D>
-int p2p_combine(const P2PCombineArgs<T> &a, long nbox, hipStream_t st);
+int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st);
 
 }  // namespace lib
 }  // namespace fmma

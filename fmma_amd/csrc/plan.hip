@@ -375,8 +375,8 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
     if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
     pl.mark(7);
     if (mutual) {
-        P2PCombineArgs<T> ca{t_start, L, pl.p2p_slots.as<T>(), nt, base, t_perm, ans, L < 2 ? 1 : 0};
-        if ((rc = p2p_combine<T, D>(ca, nbox, st))) return rc;
+        P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), nt, base, t_perm, ans, L < 2 ? 1 : 0};
+        if ((rc = p2p_combine<T, D>(ca, st))) return rc;
     }
     pl.mark(8);
     pl.last_L = L;

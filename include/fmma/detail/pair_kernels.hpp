@@ -233,15 +233,25 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
 // ------------------------------------------------------ P2P, mutual form
 //
 // Targets are the sources (the same sorted points) and f(y,x) = sym f(x,y).
-// One wave per box A, laid out as in p2p_kernel with S = 4, 8 or 16 slots.
-// The wave evaluates A against itself as the direct form does, then against
-// each neighbour B that follows A in the row-major 3^D block (the upper half,
-// (3^D-1)/2 offsets).  A staged record j of B meets each target t of the lane
-// once: w_j f(x_t, y_j) goes to t's sum and w_t f(x_t, y_j) to a partial for
-// j, which the S lanes of the slice add up by DPP row moves in a fixed order.
-// B's sums go to slots[u][j] (u = the offset's index in the upper half), A's
-// own to slots[n_upper][i]; p2p_combine (boxops.hip) adds them per point.
-constexpr int kP2PMutualTargets = 16;  // targets per lane per pass
+// One wave per box A.  The wave evaluates A against itself as the direct
+// form does, then each neighbour B that follows A in the row-major 3^D block
+// (the upper half, (3^D-1)/2 offsets) once for both directions: w_j f(x_t,y_j)
+// goes to target t's sum and sym w_t f(x_t,y_j) to B's point j.
+//
+// Lane l is (slot s = l % S, slice q = l / S), S = 4 or 16 slots: slot s
+// owns the box's targets s, s+S, ... (at most kP2PMutualTargets) in registers.
+// B's staged records are taken in chunks of S; slice q takes every (64/S)-th
+// chunk.  In step t of a chunk, lane s evaluates record (s + t) mod S of the
+// chunk and adds to a running partial sum for that record, which then moves
+// one lane down the slot ring (a DPP rotate).  After S steps each record's
+// partial has visited every slot once and sits in the lane with its index:
+// one DPP move per record and lane replaces a reduction per record.
+//
+// Output, sorted order: slots[u][j] is what point j of box B received from
+// box B - o_u (written by that box's wave, or zero by B's own wave when that
+// box is outside the grid or empty), slots[n_upper][i] point i's own sum.
+// p2p_combine (boxops.hip) adds the n_upper + 1 values of each point.
+constexpr int kP2PMutualTargets = 8;  // targets per lane per pass
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v)
@@ -258,61 +268,87 @@ __device__ __forceinline__ float dpp_move(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over aligned groups of 2^lg lanes, lg in {2, 3, 4} (within one DPP row
-// of 16 lanes); every lane of a group ends with the same value.
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int lg)
+// Lane s of each ring of S lanes gets the value of lane (s + 1) mod S.
+template <int S, typename T>
+__device__ __forceinline__ T ring_next(T v)
 {
-    v += dpp_move<0xB1>(v);                // quad_perm [1,0,3,2]: lane ^ 1
-    v += dpp_move<0x4E>(v);                // quad_perm [2,3,0,1]: lane ^ 2
-    if (lg >= 3) v += dpp_move<0x141>(v);  // row_half_mirror: lane i <-> 7 - i
-    if (lg >= 4) v += dpp_move<0x140>(v);  // row_mirror: lane i <-> 15 - i
-    return v;
+    static_assert(S == 4 || S == 16, "rings of 4 (quad) or 16 (row) lanes");
+    if constexpr (S == 4)
+        return dpp_move<0x39>(v);  // quad_perm [1,2,3,0]
+    else
+        return dpp_move<0x12F>(v);  // row_ror:15
 }
 
-template <int TPL, typename T, int D, int NT, class K>
-__device__ __forceinline__ void p2p_range_mutual(const T *src, T *rev, unsigned j0, unsigned j1, unsigned q,
-                                                 unsigned nslice, bool lead, int lg, const T (&x)[NT][D],
-                                                 const T (&wt)[NT], T (&acc)[NT], const K &kern)
+// Chunks c = q, q + nslice, ... of the mutual records starting at m.
+template <int S, int TPL, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_chunks(const T *src, T *rev, unsigned m, unsigned nch, unsigned s, unsigned q,
+                                           const T (&x)[NT][D], const T (&wt)[NT], T (&acc)[NT], const K &kern)
 {
     constexpr int RW = p2p_rec<D>::W;
-    const unsigned r0 = j0 % nslice;
-    for (unsigned j = j0 + (q >= r0 ? q - r0 : q + nslice - r0); j < j1; j += nslice) {
-        const T *r = src + j * RW;
-        T y[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) y[d] = r[d];
-        const T wj = r[D];
+    constexpr unsigned nslice = 64 / S;
+    for (unsigned c = q; c < nch; c += nslice) {
+        const unsigned rb = m + c * S;
         T part = T(0);
 #pragma unroll
-        for (int k = 0; k < TPL; ++k) {
-            const T f = pair_eval<true, T, D>(kern, x[k], y);
-            acc[k] += wj * f;
-            part += wt[k] * f;
+        for (unsigned t = 0; t < (unsigned)S; ++t) {
+            const T *r = src + (rb + ((s + t) & (S - 1))) * RW;
+            T y[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) y[d] = r[d];
+            const T wj = r[D];
+#pragma unroll
+            for (int k = 0; k < TPL; ++k) {
+                const T f = pair_eval<true, T, D>(kern, x[k], y);
+                acc[k] += wj * f;
+                part += wt[k] * f;
+            }
+            part = ring_next<S>(part);
         }
-        part = group_sum(part, lg);
-        if (lead) rev[j] = part;
+        rev[rb + s] = part;
     }
 }
 
-template <int TPL, typename T, int D, int NT, class K>
-__device__ __forceinline__ void p2p_block_mutual(const T *src, T *rev, unsigned cnt, unsigned m, unsigned q,
-                                                 unsigned nslice, bool lead, int lg, const T (&x)[NT][D],
-                                                 const T (&wt)[NT], T (&acc)[NT], const K &kern)
+template <int S, int TPL, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_block_mutual(const T *src, T *rev, unsigned m, unsigned nch, unsigned s,
+                                                 unsigned q, const T (&x)[NT][D], const T (&wt)[NT], T (&acc)[NT],
+                                                 const K &kern)
 {
-    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, nslice, x, acc, kern);
-    p2p_range_mutual<TPL>(src, rev, m, cnt, q, nslice, lead, lg, x, wt, acc, kern);
+    if (m > 0) p2p_range<TPL, false>(src, 0, m, q, 64 / S, x, acc, kern);
+    p2p_chunks<S, TPL>(src, rev, m, nch, s, q, x, wt, acc, kern);
 }
 
+template <int S, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_block_mutual_tpl(int tpl, const T *src, T *rev, unsigned m, unsigned nch,
+                                                     unsigned s, unsigned q, const T (&x)[NT][D], const T (&wt)[NT],
+                                                     T (&acc)[NT], const K &kern)
+{
+    static_assert(NT == 8, "the switch covers 1..8 targets per lane");
+    switch (tpl) {
+    case 1: p2p_block_mutual<S, 1>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 2: p2p_block_mutual<S, 2>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 3: p2p_block_mutual<S, 3>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 4: p2p_block_mutual<S, 4>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 5: p2p_block_mutual<S, 5>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 6: p2p_block_mutual<S, 6>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 7: p2p_block_mutual<S, 7>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    default: p2p_block_mutual<S, 8>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    }
+}
+
+// Three waves per SIMD (at most 168 VGPRs) measured faster than the two the
+// compiler picks by itself for the 1/r kernel: 4.69 vs 4.76 ms at 128 points
+// per box, 1.33 vs 1.68 ms at 16.
 template <typename T, int D, class K>
-__global__ __launch_bounds__(64) void p2p_mutual_kernel(fmma_p2p_args a, K kern)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2p_mutual_kernel(fmma_p2p_args a,
+                                                                                                 K kern)
 {
     constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
     constexpr int NU = (NNB - 1) / 2;
     constexpr int RW = p2p_rec<D>::W;
     constexpr int MT = kP2PMutualTargets;
-    __shared__ __align__(16) T src[kP2PBlock * RW];
-    __shared__ T rev[kP2PBlock];
+    constexpr unsigned kPad = 16;  // a chunk's padding past the block
+    __shared__ __align__(16) T src[(kP2PBlock + kPad) * RW];
+    __shared__ T rev[kP2PBlock + kPad];
     __shared__ unsigned nb_first[NU + 1], nb_pre[NU + 2];
 
     const unsigned box = blockIdx.x;
@@ -325,29 +361,40 @@ __global__ __launch_bounds__(64) void p2p_mutual_kernel(fmma_p2p_args a, K kern)
     const long n = a.n_t;
     constexpr T sym = (T)kernel_symmetry<K>::value;
 
-    // entry 0: the box itself; entry 1 + u: entry NU + 1 + u of the row-major
-    // 3^D block (last dimension fastest), whose centre is entry NU
+    // lane 0: the box itself; lane 1 + u (u < NU): the upper neighbour B =
+    // box + o_u, o_u = entry NU + 1 + u of the row-major 3^D block (last
+    // dimension fastest, centre = entry NU); lane NU + 1 + u: the lower
+    // neighbour box - o_u, whose wave writes slot u of this box's points
     unsigned cnt_nb = 0;
-    if (lane <= (unsigned)NU) {
+    bool lower_missing = false;
+    if (lane <= (unsigned)(2 * NU)) {
+        const bool upper = lane <= (unsigned)NU;
         int bc[D], m[D];
         morton_decode<D>(box, bc);
         const int NL = 1 << a.leaf_level;
         bool ok = true;
-        int k = lane == 0 ? NU : NU + (int)lane;
+        int k = lane == 0 ? NU : upper ? NU + (int)lane : (int)lane;
+        const int sgn = upper ? 1 : -1;
 #pragma unroll
         for (int d = D - 1; d >= 0; --d) {
-            m[d] = bc[d] + (k % 3) - 1;
+            m[d] = bc[d] + sgn * ((k % 3) - 1);
             k /= 3;
             ok = ok && m[d] >= 0 && m[d] < NL;
         }
-        unsigned first = 0;
+        unsigned first = 0, cnt = 0;
         if (ok) {
             const unsigned mm = morton_encode<D>(m);
             first = a.s_start[mm];
-            cnt_nb = a.s_start[mm + 1] - first;
+            cnt = a.s_start[mm + 1] - first;
         }
-        nb_first[lane] = first;
+        if (upper) {
+            cnt_nb = cnt;
+            nb_first[lane] = first;
+        } else {
+            lower_missing = cnt == 0;
+        }
     }
+    const unsigned long long missing = __ballot(lower_missing) >> (NU + 1);
     unsigned incl = cnt_nb;
 #pragma unroll
     for (int o = 1; o < 32; o <<= 1) {
@@ -356,66 +403,59 @@ __global__ __launch_bounds__(64) void p2p_mutual_kernel(fmma_p2p_args a, K kern)
     }
     if (lane <= (unsigned)NU) nb_pre[lane + 1] = incl;
     if (lane == 0) nb_pre[0] = 0;
+    // no wave of a missing (or empty) lower neighbour writes these slots
+    for (int u = 0; u < NU; ++u)
+        if ((missing >> u) & 1ull)
+            for (unsigned i = t0 + lane; i < t1; i += 64) slots[(long)u * n + i] = T(0);
     __syncthreads();
     const unsigned total = nb_pre[NU + 1], nself = nb_pre[1];
 
-    // slots: the fewest of 4, 8, 16 with <= MT targets per lane (more passes beyond)
+    // ring size: 4 slots for boxes of <= 4 MT points, else 16 with balanced passes
     const unsigned ntb = t1 - t0;
-    int lg = 2;
-    while (lg < 4 && (1u << lg) * MT < ntb) ++lg;
-    const unsigned nslot = 1u << lg, nslice = 64 / nslot, s = lane % nslot, q = lane / nslot;
-    const bool lead = s == 0;
+    const bool small = ntb <= 4u * MT;
+    const unsigned S = small ? 4u : 16u, s = lane % S, q = lane / S;
+    const unsigned npasses = small ? 1u : (ntb + 16u * MT - 1) / (16u * MT);
+    const unsigned per = (ntb + npasses - 1) / npasses;
 
-    for (unsigned pass0 = t0; pass0 < t1; pass0 += nslot * MT) {
-        const unsigned npass = t1 - pass0 < nslot * MT ? t1 - pass0 : nslot * MT;
-        const int tpl = (int)((npass + nslot - 1) / nslot);
+    for (unsigned pass = 0; pass < npasses; ++pass) {
+        const unsigned p0 = t0 + pass * per, p1 = p0 + per < t1 ? p0 + per : t1;
+        if (p0 >= t1) break;
+        const int tpl = (int)((p1 - p0 + S - 1) / S);
         T x[MT][D], wt[MT], acc[MT];
 #pragma unroll
         for (int k = 0; k < MT; ++k) {
             acc[k] = T(0);
-            const unsigned i = pass0 + s + nslot * k;
-            const unsigned ic = i < t1 ? i : t1 - 1;
+            const unsigned i = p0 + s + S * k;
+            const unsigned ic = i < p1 ? i : p1 - 1;
 #pragma unroll
             for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? pos[d * n + ic] : T(0);
-            wt[k] = k < tpl && i < t1 ? pw[i] : T(0);  // padding lanes add nothing to B
+            wt[k] = k < tpl && i < p1 ? pw[i] : T(0);  // padding targets add nothing to B
         }
         for (unsigned base = 0; base < total; base += kP2PBlock) {
             const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
-            __syncthreads();  // previous block fully consumed
+            const unsigned m = nself > base ? (nself - base < cnt ? nself - base : cnt) : 0u;
+            const unsigned nch = (cnt - m + S - 1) / S;
+            const unsigned cpad = m + nch * S;  // <= cnt + S - 1
+            __syncthreads();                    // previous block fully consumed
             int b = 0;
-            for (unsigned e = lane; e < cnt; e += 64) {
-                const unsigned p = base + e;
+            for (unsigned e = lane; e < cpad; e += 64) {
+                // records past cnt repeat record cnt - 1 (of a neighbour box) with weight 0
+                const unsigned p = base + (e < cnt ? e : cnt - 1);
                 while (nb_pre[b + 1] <= p) ++b;
                 const long j = (long)nb_first[b] + (p - nb_pre[b]);
                 T *r = src + e * RW;
 #pragma unroll
                 for (int d = 0; d < D; ++d) r[d] = pos[d * n + j];
-                r[D] = pw[j];
+                r[D] = e < cnt ? pw[j] : T(0);
             }
             __syncthreads();
-            const unsigned m = nself > base ? (nself - base < cnt ? nself - base : cnt) : 0u;
-            switch (tpl) {
-            case 1: p2p_block_mutual<1>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 2: p2p_block_mutual<2>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 3: p2p_block_mutual<3>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 4: p2p_block_mutual<4>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 5: p2p_block_mutual<5>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 6: p2p_block_mutual<6>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 7: p2p_block_mutual<7>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 8: p2p_block_mutual<8>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 9: p2p_block_mutual<9>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 10: p2p_block_mutual<10>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 11: p2p_block_mutual<11>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 12: p2p_block_mutual<12>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 13: p2p_block_mutual<13>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 14: p2p_block_mutual<14>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            case 15: p2p_block_mutual<15>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            default: p2p_block_mutual<16>(src, rev, cnt, m, q, nslice, lead, lg, x, wt, acc, kern); break;
-            }
+            if (small)
+                p2p_block_mutual_tpl<4>(tpl, src, rev, m, nch, s, q, x, wt, acc, kern);
+            else
+                p2p_block_mutual_tpl<16>(tpl, src, rev, m, nch, s, q, x, wt, acc, kern);
             __syncthreads();
             // the neighbours' sums of this block (records m .. cnt): the first
-            // pass stores, later passes add (the same thread owns an entry in
-            // every pass)
+            // pass stores, later passes add (one thread owns an entry in every pass)
             b = 0;
             for (unsigned e = lane; e < cnt; e += 64) {
                 if (e < m) continue;
@@ -424,23 +464,23 @@ __global__ __launch_bounds__(64) void p2p_mutual_kernel(fmma_p2p_args a, K kern)
                 const long j = (long)nb_first[b] + (p - nb_pre[b]);
                 T *dst = slots + (long)(b - 1) * n + j;
                 const T v = sym * rev[e];
-                *dst = pass0 == t0 ? v : *dst + v;
+                *dst = pass == 0 ? v : *dst + v;
             }
         }
-        // the box's own sums: add the slices of each target (lanes s + nslot q')
+        // the box's own sums: add the slices of each target (lanes s + S q')
 #pragma unroll
         for (int k = 0; k < MT; ++k) {
             if (k < tpl) {
                 T v = acc[k];
-                for (unsigned o = nslot; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
+                for (unsigned o = S; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
                 acc[k] = v;
             }
         }
         if (q == 0) {
 #pragma unroll
             for (int k = 0; k < MT; ++k) {
-                const unsigned i = pass0 + s + nslot * k;
-                if (k < tpl && i < t1) slots[(long)NU * n + i] = acc[k];
+                const unsigned i = p0 + s + S * k;
+                if (k < tpl && i < p1) slots[(long)NU * n + i] = acc[k];
             }
         }
     }

@@ -83,10 +83,11 @@ typedef struct {
      * sorted points (t_pos == s_pos, t_start == s_start) and f(y,x) =
      * symmetry * f(x,y).  Box A evaluates itself and each neighbour B that
      * follows it in the row-major 3^dim block once, adding to A's sums and,
-     * times symmetry, to B's.  The launcher writes slots[u][i] (u < n_upper =
-     * (3^dim-1)/2: what point i of box B received from box B - o_u, the u-th
-     * such offset) and slots[n_upper][i] (point i's own sum), sorted order,
-     * and leaves ans alone; fmma's combine pass adds them in a fixed order. */
+     * times symmetry, to B's.  The launcher writes every slots[u][i] (u <
+     * n_upper = (3^dim-1)/2: what point i of box B received from box B - o_u,
+     * o_u the u-th such offset, or 0 when that box is outside the grid or
+     * empty) and slots[n_upper][i] (point i's own sum), sorted order, and
+     * leaves ans alone; the library's combine pass adds them in a fixed order. */
     void *slots;               /* [n_upper+1][n_t] TYPE, or NULL for the direct form */
     int symmetry;              /* +1 or -1 in the mutual form */
 } fmma_p2p_args;
