@@ -26,6 +26,14 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
+def _dptr(x):
+    return x.data_ptr() if x is not None else None
+
+
+def _hptr(x):
+    return x.ctypes.data if x is not None else None
+
+
 class Plan:
     """One FMM problem configuration: ``fmma::FMMA<TYPE, DIM>``.
 
@@ -145,6 +153,75 @@ class Plan:
         ptr = lambda x: x.ctypes.data if x is not None else None  # noqa: E731
         _lib.check(self._L.fmma_plan_solve_host(self._h, t.shape[0], ptr(t), ptr(w), s.shape[0], ptr(s), ptr(a)))
         return a
+
+    # ---- differentiation (new; the reference has no gradients)
+    def gradient(self, targets, weights, sources, grad, reuse=True):
+        """grad[i, d] <- d ans_i / d x_{i,d} for ans = solve(targets, weights,
+        sources): the derivative of the fmm approximation (L2P interpolant
+        differentiated, plus the near field of grad_x f), or of the exact sum
+        for the exact solvers.  With reuse the local expansions of this plan's
+        previous fmm solve are used when it had the same arrays and settings
+        (the caller guarantees their contents are unchanged)."""
+        self._check_vec(targets, weights, sources, grad, (targets.shape[0], self.dim), "grad")
+        if _is_torch(targets):
+            self._torch_ready(targets, weights, sources, grad)
+            _lib.check(self._L.fmma_plan_gradient(self._h, targets.shape[0], targets.data_ptr(), _dptr(weights),
+                                                  sources.shape[0], sources.data_ptr(), grad.data_ptr(),
+                                                  1 if reuse else 0))
+        else:
+            self._numpy_ready(targets, weights, sources, grad)
+            _lib.check(self._L.fmma_plan_gradient_host(self._h, targets.shape[0], _hptr(targets), _hptr(weights),
+                                                       sources.shape[0], _hptr(sources), _hptr(grad)))
+        return grad
+
+    def solve_transpose(self, targets, cot, sources, out):
+        """out[j] <- sum_i cot_i f(x_i, y_j): the adjoint in the weights of
+        solve(targets, ., sources), i.e. d(sum_i cot_i ans_i)/d w_j.  The fmm
+        solver runs with the roles swapped and f transposed on the same tree
+        and Depth, so it is the exact transpose of that solve's operator."""
+        if cot is None or cot.ndim != 1 or cot.shape[0] != targets.shape[0]:
+            raise ValueError("cot must have shape (N,)")
+        self._check_vec(targets, None, sources, out, (sources.shape[0],), "out")
+        if _is_torch(targets):
+            self._torch_ready(targets, cot, sources, out)
+            _lib.check(self._L.fmma_plan_solve_transpose(self._h, targets.shape[0], targets.data_ptr(),
+                                                         cot.data_ptr(), sources.shape[0], sources.data_ptr(),
+                                                         out.data_ptr()))
+        else:
+            self._numpy_ready(targets, cot, sources, out)
+            _lib.check(self._L.fmma_plan_solve_transpose_host(self._h, targets.shape[0], _hptr(targets), _hptr(cot),
+                                                              sources.shape[0], _hptr(sources), _hptr(out)))
+        return out
+
+    def _check_vec(self, t, w, s, o, shape, name):
+        if t.ndim != 2 or t.shape[1] != self.dim:
+            raise ValueError("targets must have shape (N, %d)" % self.dim)
+        if s.ndim != 2 or s.shape[1] != self.dim:
+            raise ValueError("sources must have shape (M, %d)" % self.dim)
+        if w is not None and (w.ndim != 1 or w.shape[0] != s.shape[0]):
+            raise ValueError("weights must have shape (M,)")
+        if tuple(o.shape) != tuple(shape):
+            raise ValueError("%s must have shape %s" % (name, tuple(shape)))
+
+    def _torch_ready(self, *xs):
+        import torch
+        want = torch.float64 if self.dtype == np.float64 else torch.float32
+        for x in xs:
+            if x is None:
+                continue
+            if not x.is_cuda or x.dtype != want or not x.is_contiguous():
+                raise ValueError("torch inputs must be contiguous %s tensors on the GPU" % want)
+        self.set_stream(torch.cuda.current_stream(xs[-1].device).cuda_stream)
+
+    def _numpy_ready(self, *xs):
+        for x in xs:
+            if x is None:
+                continue
+            if not isinstance(x, np.ndarray) or x.dtype != self.dtype or not x.flags.c_contiguous:
+                raise ValueError("numpy inputs must be C-contiguous %s arrays" % self.dtype)
+        if not xs[-1].flags.writeable:
+            raise ValueError("the output must be writeable")
+        self.set_stream(0)
 
     # ---- diagnostics
     def time_log(self):

@@ -515,6 +515,78 @@ static bool l2p_fast(const L2PArgs<T> &a, hipStream_t st)
     FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((l2p_ct<T, D, NN>), dim3(nb), dim3(256), 0, st, a));
 }
 
+// ----------------------------------------------------------- L2P gradient
+//
+// d/dx_d of the L2P interpolant u(x) = sum_k Wl[b][k] prod_e S_n(r_e, c_{k_e})
+// (src/fmm_core.cpp:602-612), r_e = 2((x_e - o_e)/len - b_e) - 1: the
+// Chebyshev basis differentiated, S_n'(r, c_k) = sum_j coef[k][j] T_j'(r) with
+// T_j' = j U_{j-1} (U from its three-term recurrence), times dr/dx = 2/len.
+// One thread per sorted target; out is [D][n] in sorted order.
+
+template <typename T, int D, int NMAX>
+__global__ __launch_bounds__(256) void l2p_grad_kernel(L2PArgs<T> a)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int n = a.nn;
+    const unsigned box = a.keys[i];
+    const T *W = a.Wl + (size_t)box * a.P;
+    const int NL = 1 << a.leaf_level;
+    const T len = a.geom->len_leaf;
+    T S[D][NMAX], dS[D][NMAX];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        T r;
+        leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, r);
+        for (int k = 0; k < n; ++k) {
+            S[d][k] = a.coef[k * n];
+            dS[d][k] = T(0);
+        }
+        T tp = T(1), t = r;        // T_{j-1}, T_j
+        T up = T(0), u = T(1);     // U_{j-2}, U_{j-1}
+        for (int j = 1; j < n; ++j) {
+            const T dt = T(j) * u;  // T_j'
+            for (int k = 0; k < n; ++k) {
+                S[d][k] += a.coef[k * n + j] * t;
+                dS[d][k] += a.coef[k * n + j] * dt;
+            }
+            const T tn = T(2) * r * t - tp;
+            tp = t;
+            t = tn;
+            const T un = T(2) * r * u - up;
+            up = u;
+            u = un;
+        }
+    }
+    T g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = T(0);
+    if constexpr (D == 1) {
+        for (int a0 = 0; a0 < n; ++a0) g[0] += W[a0] * dS[0][a0];
+    } else if constexpr (D == 2) {
+        for (int a0 = 0; a0 < n; ++a0)
+            for (int a1 = 0; a1 < n; ++a1) {
+                const T w = W[a0 * n + a1];
+                g[0] += w * (dS[0][a0] * S[1][a1]);
+                g[1] += w * (S[0][a0] * dS[1][a1]);
+            }
+    } else {
+        for (int a0 = 0; a0 < n; ++a0)
+            for (int a1 = 0; a1 < n; ++a1) {
+                const T s01 = S[0][a0] * S[1][a1], d01 = dS[0][a0] * S[1][a1], s0d1 = S[0][a0] * dS[1][a1];
+                for (int a2 = 0; a2 < n; ++a2) {
+                    const T w = W[(a0 * n + a1) * n + a2];
+                    g[0] += w * (d01 * S[2][a2]);
+                    g[1] += w * (s0d1 * S[2][a2]);
+                    g[2] += w * (s01 * dS[2][a2]);
+                }
+            }
+    }
+    const T scale = T(2) / len;
+#pragma unroll
+    for (int d = 0; d < D; ++d) a.out[d * a.n + i] = g[d] * scale;
+}
+
 // ------------------------------------------------------------ P2P combine
 //
 // Final sums of the mutual P2P form: point i's own sum, then slot u = 0 ..
@@ -593,6 +665,39 @@ int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st)
 }
 
 template <typename T, int D>
+int l2p_grad(const L2PArgs<T> &a, hipStream_t st)
+{
+    if (a.n <= 0) return FMMA_OK;
+    const unsigned nb = (unsigned)((a.n + 255) / 256);
+    if (a.nn <= 8)
+        hipLaunchKernelGGL((l2p_grad_kernel<T, D, 8>), dim3(nb), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((l2p_grad_kernel<T, D, kMaxNodes>), dim3(nb), dim3(256), 0, st, a);
+    FMMA_LAUNCHED("l2p_grad");
+    return FMMA_OK;
+}
+
+// out[i] = s * out[i] (the sign of an antisymmetric kernel's transpose), or
+// out[i] = 0 with s == 0.
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(T *out, long n, T s)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = s == T(0) ? T(0) : s * out[i];
+}
+
+template <typename T>
+int scale(T *out, long n, T s, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    hipLaunchKernelGGL((scale_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, s);
+    FMMA_LAUNCHED("scale");
+    return FMMA_OK;
+}
+template int scale<double>(double *, long, double, hipStream_t);
+template int scale<float>(float *, long, float, hipStream_t);
+
+template <typename T, int D>
 int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st)
 {
     if (a.n <= 0) return FMMA_OK;
@@ -602,6 +707,7 @@ int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st)
 }
 
 #define FMMA_BOX_INST(T, D)                                                                 \
+    template int l2p_grad<T, D>(const L2PArgs<T> &, hipStream_t);                           \
     template int p2p_combine<T, D>(const P2PCombineArgs<T> &, hipStream_t);                 \
     template int p2m<T, D>(const P2MArgs<T> &, long, hipStream_t);                          \
     template int m2m<T, D>(const T *, T *, long, const T *, int, int, hipStream_t);         \

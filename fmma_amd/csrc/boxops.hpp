@@ -60,5 +60,14 @@ int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);
 template <typename T, int D>
 int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st);
 
+// Gradient of the L2P interpolant with respect to the target coordinates:
+// a.out is [D][n], sorted order.
+template <typename T, int D>
+int l2p_grad(const L2PArgs<T> &a, hipStream_t st);
+
+// out[i] *= s for i < n, or out[i] = 0 when s == 0.
+template <typename T>
+int scale(T *out, long n, T s, hipStream_t st);
+
 }  // namespace lib
 }  // namespace fmma

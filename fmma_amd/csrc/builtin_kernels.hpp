@@ -10,6 +10,9 @@
 // in floating point too (x - y and y - x differ only in sign); P2P uses it
 // to evaluate each pair of distinct neighbour boxes once.
 //
+// grad(x, y, g) writes grad_x f(x, y), the near-field part of the FMM
+// gradient; grad_far() is its form without the r = 0 case where there is one.
+//
 // Kernels that special-case r = 0 also provide far(): the same function
 // without that test, which P2P uses for the 3^D - 1 neighbour boxes other
 // than the target's own box (points in different boxes never coincide).
@@ -64,6 +67,24 @@ struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.h
     {
         return rsqrt_refined(dist2<T, D>(x, y));
     }
+    // grad_x f = -(x - y) / r^3; taken as 0 at r = 0, where f is infinite
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T inv = rsqrt_refined(r2);
+        const T c = r2 == T(0) ? T(0) : -(inv * inv * inv);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ void grad_far(const T *x, const T *y, T *g) const
+    {
+        const T inv = rsqrt_refined(dist2<T, D>(x, y));
+        const T c = -(inv * inv * inv);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
 };
 
 struct KInvRReg {  // 1/|x-y|, 0 at r = 0
@@ -82,6 +103,23 @@ struct KInvRReg {  // 1/|x-y|, 0 at r = 0
     {
         return rsqrt_refined(dist2<T, D>(x, y));
     }
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T inv = rsqrt_refined(r2);
+        const T c = r2 == T(0) ? T(0) : -(inv * inv * inv);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ void grad_far(const T *x, const T *y, T *g) const
+    {
+        const T inv = rsqrt_refined(dist2<T, D>(x, y));
+        const T c = -(inv * inv * inv);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
 };
 
 struct KInvDiff {  // 1/(x_0 - y_0)
@@ -92,6 +130,15 @@ struct KInvDiff {  // 1/(x_0 - y_0)
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
     {
         return T(1) / (x[0] - y[0]);
+    }
+    // d/dx_0 = -1 / (x_0 - y_0)^2, the other components 0
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T inv = T(1) / (x[0] - y[0]);
+        g[0] = -(inv * inv);
+#pragma unroll
+        for (int d = 1; d < D; ++d) g[d] = T(0);
     }
 };
 
@@ -110,6 +157,15 @@ struct KLogR {  // log|x-y| = 0.5 log r^2, 0 at r = 0
     {
         return T(0.5) * std::log(dist2<T, D>(x, y));
     }
+    // grad_x f = (x - y) / r^2, 0 at r = 0
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T c = r2 == T(0) ? T(0) : T(1) / r2;
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
 };
 
 struct KGauss {  // exp(-r^2 / (2 h^2)); p1 = 1 / (2 h^2)
@@ -121,6 +177,15 @@ struct KGauss {  // exp(-r^2 / (2 h^2)); p1 = 1 / (2 h^2)
     {
         const T r2 = dist2<T, D>(x, y);
         return std::exp(-r2 * (T)p1);
+    }
+    // grad_x f = -2 p1 (x - y) f
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T c = T(-2) * (T)p1 * std::exp(-r2 * (T)p1);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
     }
 };
 
@@ -143,6 +208,29 @@ struct KYukawa {  // exp(-kappa r) / r, 0 at r = 0; p0 = kappa
         const T inv = rsqrt_refined(r2);
         return std::exp(-(T)p0 * (r2 * inv)) * inv;
     }
+    // grad_x f = -(1 + kappa r) exp(-kappa r) (x - y) / r^3, 0 at r = 0
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T c = r2 == T(0) ? T(0) : coef(r2);
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ void grad_far(const T *x, const T *y, T *g) const
+    {
+        const T c = coef(dist2<T, D>(x, y));
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = c * (x[d] - y[d]);
+    }
+    template <typename T>
+    __device__ __forceinline__ T coef(T r2) const
+    {
+        const T inv = rsqrt_refined(r2);
+        const T kr = (T)p0 * (r2 * inv);
+        return -(T(1) + kr) * std::exp(-kr) * (inv * inv * inv);
+    }
 };
 
 struct KSumR2 {  // sum_d (x_d - y_d) / r^2
@@ -161,6 +249,21 @@ struct KSumR2 {  // sum_d (x_d - y_d) / r^2
         }
         return s / r2;
     }
+    // d/dx_d = 1 / r^2 - 2 s (x_d - y_d) / r^4, s = sum_e (x_e - y_e)
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        T r2 = T(0), s = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T t = x[d] - y[d];
+            r2 = __builtin_fma(t, t, r2);
+            s += t;
+        }
+        const T i2 = T(1) / r2, c = T(-2) * s * i2 * i2;
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = i2 + c * (x[d] - y[d]);
+    }
 };
 
 struct KSum2R2 {  // sum_d (x_d + y_d)^2 / r^2: depends on x and y, not only x - y
@@ -178,6 +281,21 @@ struct KSum2R2 {  // sum_d (x_d + y_d)^2 / r^2: depends on x and y, not only x -
             s2 = __builtin_fma(u, u, s2);
         }
         return s2 / r2;
+    }
+    // d/dx_d = 2 (x_d + y_d) / r^2 - 2 S (x_d - y_d) / r^4, S = sum_e (x_e + y_e)^2
+    template <typename T, int D>
+    __device__ __forceinline__ void grad(const T *x, const T *y, T *g) const
+    {
+        T r2 = T(0), s2 = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T t = x[d] - y[d], u = x[d] + y[d];
+            r2 = __builtin_fma(t, t, r2);
+            s2 = __builtin_fma(u, u, s2);
+        }
+        const T i2 = T(1) / r2, c = T(-2) * s2 * i2 * i2;
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = T(2) * (x[d] + y[d]) * i2 + c * (x[d] - y[d]);
     }
 };
 

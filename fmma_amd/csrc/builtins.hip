@@ -32,6 +32,14 @@ struct builtin_ops {
     {
         return detail::launch_m2l_general<T, D, K>(a, make(ctx), (hipStream_t)s);
     }
+    static int p2p_grad(const fmma_p2p_grad_args *a, void *ctx, void *s)
+    {
+        return detail::launch_p2p_grad<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
+    static int exact_grad(const fmma_exact_grad_args *a, void *ctx, void *s)
+    {
+        return detail::launch_exact_grad<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
     static void fill(fmma_kernel_ops *o, void *ctx)
     {
         o->dim = D;
@@ -43,6 +51,9 @@ struct builtin_ops {
         o->exact = &exact;
         o->m2l_op = &m2l_op;
         o->m2l_general = &m2l_general;
+        o->p2p_grad = &p2p_grad;
+        o->exact_grad = &exact_grad;
+        o->transposed = nullptr;  // every built-in is symmetric or antisymmetric
     }
 };
 

@@ -202,6 +202,10 @@ __device__ __forceinline__ void target_coords(const M2LTask &t, int tid, int (&t
     for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((t.cls >> (D - 1 - d)) & 1);
 }
 
+// Operand rows staged per LDS pass of m2l_mfma4: 64 boxes x (256 + 1) FP64
+// values = 132 KB, inside the 160 KB of a CU for every P <= 512.
+constexpr int kM2LChunk = 256;
+
 // MODE 0: FP64 MFMA on FP64 data.  MODE 1: FP32 MFMA, per-offset FP32 sums
 // added into TW accumulators.  TW is the storage type of Wm/Wl.
 template <typename TW, int D, int MODE>
@@ -217,7 +221,8 @@ __global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int rb = blockIdx.y * 4 + wave;
     const bool active = rb * 16 < a.MP;
-    const int N = 1 << task.level, P = a.P, KP = a.KP, ldb = KP + 1;
+    const int N = 1 << task.level, P = a.P, KP = a.KP;
+    const int KC = KP < kM2LChunk ? KP : kM2LChunk, ldb = KC + 1;
     int tc[D];
     bool tvalid;
     target_coords<D>(task, tid, tc, tvalid);
@@ -236,31 +241,38 @@ __global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
         const int f = a.cls_f[task.cls * a.n_cls + i];
         partner_indices<D>(a, task.cls, i, N, tc, tvalid, pidx, tid);
         __syncthreads();
-        for (int e = tid; e < 64 * KP; e += 256) {
-            const int col = e / KP, k = e - col * KP;
-            const int pi = pidx[col];
-            CT v = CT(0);
-            if (pi >= 0 && k < P) v = (CT)Wm[(size_t)pi * P + k];
-            Bs[col * ldb + k] = v;
-        }
-        __syncthreads();
-        if (active) {
-            const CT *A = Kf + ((size_t)f * nrb + rb) * nks * 64;
-            V4 c[4];
+        const CT *A = Kf + ((size_t)f * nrb + rb) * nks * 64;
+        V4 c[4];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) c[cb] = V4{0, 0, 0, 0};
-            for (int s = 0; s < nks; ++s) {
-                const CT av = A[s * 64 + lane];
-                const int kk = 4 * s + (lane >> 4);
+        for (int cb = 0; cb < 4; ++cb) c[cb] = V4{0, 0, 0, 0};
+        // the contraction in chunks of KC operand rows staged in LDS
+        for (int k0 = 0; k0 < KP; k0 += KC) {
+            const int kc = KP - k0 < KC ? KP - k0 : KC;
+            for (int e = tid; e < 64 * kc; e += 256) {
+                const int col = e / kc, k = e - col * kc;
+                const int pi = pidx[col];
+                CT v = CT(0);
+                if (pi >= 0 && k0 + k < P) v = (CT)Wm[(size_t)pi * P + k0 + k];
+                Bs[col * ldb + k] = v;
+            }
+            __syncthreads();
+            if (active) {
+                for (int s = k0 / 4; s < (k0 + kc) / 4; ++s) {
+                    const CT av = A[s * 64 + lane];
+                    const int kk = 4 * s + (lane >> 4) - k0;
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) {
-                    const CT bv = Bs[(16 * cb + (lane & 15)) * ldb + kk];
-                    if constexpr (MODE == 0)
-                        c[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c[cb], 0, 0, 0);
-                    else
-                        c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c[cb], 0, 0, 0);
+                    for (int cb = 0; cb < 4; ++cb) {
+                        const CT bv = Bs[(16 * cb + (lane & 15)) * ldb + kk];
+                        if constexpr (MODE == 0)
+                            c[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c[cb], 0, 0, 0);
+                        else
+                            c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c[cb], 0, 0, 0);
+                    }
                 }
             }
+            __syncthreads();  // the chunk is consumed before the next is staged
+        }
+        if (active) {
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
@@ -595,7 +607,7 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
             if constexpr (std::is_same<T, double>::value) {
                 done = g.pipelined && launch_pipe<T, D, 0>(a, g.tb, grid, st);
                 if (!done) {
-                    const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(double);
+                    const size_t lds = (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + 1) * sizeof(double);
                     hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
                 }
             } else {
@@ -604,7 +616,7 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
         } else if (mode == 1) {
             done = g.pipelined && launch_pipe<T, D, 1>(a, g.tb, grid, st);
             if (!done) {
-                const size_t lds = (size_t)64 * (a.KP + 1) * sizeof(float);
+                const size_t lds = (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + 1) * sizeof(float);
                 hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
             }
         } else {

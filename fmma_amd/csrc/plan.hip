@@ -112,6 +112,21 @@ struct Plan {
     int last_L = -1, last_shared = 0;
     long last_nt = 0, last_ns = 0;
 
+    // The tree and local expansions of the last fmm passes, for a gradient
+    // that reuses them: valid while the inputs and settings are unchanged.
+    struct View {
+        bool valid = false;
+        const void *tg = nullptr, *w = nullptr, *sr = nullptr;
+        long nt = 0, ns = 0;
+        unsigned long gen = 0;
+        int L = -1;
+        bool shared = false;
+        const void *t_pos = nullptr;
+        const unsigned *t_start = nullptr, *t_perm = nullptr, *t_keys = nullptr;
+    } view;
+    unsigned long gen = 1;  // bumped by every setter that changes results
+    DBuf grad_far;
+
     ~Plan()
     {
         if (events_ready)
@@ -252,9 +267,11 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
     return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, pl.stream);
 }
 
+// Tree, P2M, M2M, M2L and L2L (stages 0-4); records pl.view.
 template <typename T, int D>
-static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr)
 {
+    pl.view.valid = false;
     const int L = pl.depth - 1;
     if (D * L > 30 || L >= kMaxLevels)
         return fail(FMMA_ERR_UNSUPPORTED, "Depth " + std::to_string(pl.depth) + " too deep for dimension " +
@@ -354,7 +371,41 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
         if ((rc = l2l<T, D>(Wl + pl.lvl_off[l], Wl + pl.lvl_off[l + 1], 1L << (D * l), pl.d_tab.as<T>(), n, P, st)))
             return rc;
     pl.mark(5);
+    Plan::View &v = pl.view;
+    v.tg = tg;
+    v.w = w;
+    v.sr = sr;
+    v.nt = nt;
+    v.ns = ns;
+    v.gen = pl.gen;
+    v.L = L;
+    v.shared = shared;
+    v.t_pos = t_pos;
+    v.t_start = t_start;
+    v.t_perm = t_perm;
+    v.t_keys = t_keys;
+    v.valid = true;
+    pl.last_L = L;
+    pl.last_shared = shared ? 1 : 0;
+    pl.last_nt = nt;
+    pl.last_ns = ns;
+    return FMMA_OK;
+}
 
+// L2P and P2P (stages 5-7) on the passes recorded in pl.view.
+template <typename T, int D>
+static int fmm_eval(Plan &pl, T *ans)
+{
+    const Plan::View &v = pl.view;
+    const int L = v.L, n = pl.poly_ord + 1, P = pl.P();
+    const long nt = v.nt, ns = v.ns;
+    const bool shared = v.shared;
+    const unsigned nbox = 1u << (D * L);
+    const T *t_pos = static_cast<const T *>(v.t_pos);
+    const unsigned *t_start = v.t_start, *t_perm = v.t_perm, *t_keys = v.t_keys;
+    T *Wl = pl.Wl.as<T>();
+    hipStream_t st = pl.stream;
+    int rc;
     const T *base = nullptr;
     if (L >= 2) {
         if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
@@ -379,11 +430,39 @@ static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T 
         if ((rc = p2p_combine<T, D>(ca, st))) return rc;
     }
     pl.mark(8);
-    pl.last_L = L;
-    pl.last_shared = shared ? 1 : 0;
-    pl.last_nt = nt;
-    pl.last_ns = ns;
     return FMMA_OK;
+}
+
+template <typename T, int D>
+static int run_fmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+{
+    int rc;
+    if ((rc = fmm_passes<T, D>(pl, nt, tg, w, ns, sr))) return rc;
+    return fmm_eval<T, D>(pl, ans);
+}
+
+// Gradient of the fmm result with respect to the target coordinates: the
+// L2P interpolant differentiated, plus the near field of grad_x f.
+template <typename T, int D>
+static int fmm_eval_grad(Plan &pl, T *grad)
+{
+    const Plan::View &v = pl.view;
+    const int L = v.L, n = pl.poly_ord + 1, P = pl.P();
+    const long nt = v.nt, ns = v.ns;
+    const unsigned nbox = 1u << (D * L);
+    hipStream_t st = pl.stream;
+    int rc;
+    const T *base = nullptr;
+    if (L >= 2) {
+        if ((rc = pl.grad_far.ensure((size_t)D * nt * sizeof(T)))) return rc;
+        L2PArgs<T> la{v.t_start, v.t_keys, static_cast<const T *>(v.t_pos), nt, pl.geom.as<Geom<T>>(), L,
+                      pl.d_coef.as<T>(), n, P, pl.Wl.as<T>() + pl.lvl_off[L], pl.grad_far.as<T>()};
+        if ((rc = l2p_grad<T, D>(la, st))) return rc;
+        base = pl.grad_far.as<T>();
+    }
+    fmma_p2p_grad_args ga{D, pl.dtype, (long)nbox, L, v.t_start, pl.s_start.as<unsigned>(), v.t_pos,
+                          pl.s_pos.as<T>(), pl.s_w.as<T>(), nt, ns, base, v.t_perm, grad};
+    return pl.ops.p2p_grad(&ga, pl.ops.ctx, st);
 }
 
 template <typename T, int D>
@@ -404,16 +483,47 @@ static int run_exact(Plan &pl, long nt, const T *tg, const T *w, long ns, const 
     return pl.ops.exact(&ea, pl.ops.ctx, pl.stream);
 }
 
+enum class Op { Solve, Gradient, Transpose };
+
+// out = the solve's ans [n_t], the gradient [n_t][D], or the transpose [n_s].
 template <typename T, int D>
-static int solve_typed(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans,
-                       bool is_fmm)
+static int op_typed(Plan &pl, Op op, long nt, const void *tg_, const void *w_, long ns, const void *sr_, void *out_,
+                    bool is_fmm, bool reuse)
 {
-    if (is_fmm)
-        return run_fmm<T, D>(pl, nt, (const T *)tg, (const T *)w, ns, (const T *)sr, (T *)ans);
-    return run_exact<T, D>(pl, nt, (const T *)tg, (const T *)w, ns, (const T *)sr, (T *)ans);
+    const T *tg = (const T *)tg_, *w = (const T *)w_, *sr = (const T *)sr_;
+    T *out = (T *)out_;
+    int rc;
+    if (op == Op::Solve)
+        return is_fmm ? run_fmm<T, D>(pl, nt, tg, w, ns, sr, out) : run_exact<T, D>(pl, nt, tg, w, ns, sr, out);
+    if (op == Op::Gradient) {
+        if (!is_fmm) {
+            fmma_exact_grad_args ea{D, pl.dtype, nt, ns, tg, w, sr, out};
+            return pl.ops.exact_grad(&ea, pl.ops.ctx, pl.stream);
+        }
+        const Plan::View &v = pl.view;
+        const bool same = reuse && v.valid && v.tg == tg_ && v.w == w_ && v.sr == sr_ && v.nt == nt && v.ns == ns &&
+                          v.gen == pl.gen && v.L == pl.depth - 1;
+        if (!same && (rc = fmm_passes<T, D>(pl, nt, tg, w, ns, sr))) return rc;
+        return fmm_eval_grad<T, D>(pl, out);
+    }
+    // transpose: targets and sources swap roles, f becomes f^T
+    const fmma_kernel_ops saved = pl.ops;
+    T sgn = T(1);
+    if (saved.symmetry == 1 || saved.symmetry == -1)
+        sgn = (T)saved.symmetry;
+    else if (saved.transposed)
+        pl.ops = *saved.transposed;
+    else
+        return fail(FMMA_ERR_UNSUPPORTED, "the kernel declares no symmetry and has no transposed launcher table");
+    if (is_fmm && pl.depth - 1 < 2 && (rc = scale<T>(out, ns, T(0), pl.stream))) return rc;  // P2P adds onto out
+    rc = is_fmm ? run_fmm<T, D>(pl, ns, sr, w, nt, tg, out) : run_exact<T, D>(pl, ns, sr, w, nt, tg, out);
+    pl.ops = saved;
+    if (!rc && sgn != T(1)) rc = scale<T>(out, ns, sgn, pl.stream);
+    return rc;
 }
 
-static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans)
+static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long ns, const void *sr, void *out,
+                  bool reuse)
 {
     const std::string &s = pl.solver;
     const bool is_fmm = s == "fmm";
@@ -424,45 +534,66 @@ static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long n
     }
     if (nt < 0 || ns < 0) return fail(FMMA_ERR_INVALID, "negative size");
     if (is_fmm && pl.depth <= 0) {
-        // auto Depth from the target count, kept afterwards (src/fmm.cpp:23-28)
+        // auto Depth from the target count, kept afterwards (src/fmm.cpp:23-28); the
+        // transpose and the gradient use the Depth of the solve they belong to
         pl.depth = (int)(std::log((double)nt) / pl.dim) + 1;
         if (pl.depth <= 0) pl.depth = 1;
     }
-    if (nt == 0 || ns == 0) return FMMA_OK;
-    if (!tg || !sr || !ans) return fail(FMMA_ERR_INVALID, "null pointer");
+    if (op == Op::Gradient && (!pl.ops.p2p_grad || !pl.ops.exact_grad))
+        return fail(FMMA_ERR_UNSUPPORTED, "the kernel has no gradient");
+    if (nt == 0 || ns == 0) {
+        // an empty sum: the gradient and the transpose are zero (the solve leaves ans alone)
+        if (op == Op::Solve) return FMMA_OK;
+        const long m = op == Op::Gradient ? nt * pl.dim : ns;
+        if (m == 0) return FMMA_OK;
+        if (!out) return fail(FMMA_ERR_INVALID, "null pointer");
+        const int rc = pl.dtype == FMMA_F64 ? scale<double>((double *)out, m, 0.0, pl.stream)
+                                            : scale<float>((float *)out, m, 0.f, pl.stream);
+        if (rc) return rc;
+        FMMA_HIP(hipStreamSynchronize(pl.stream));
+        return FMMA_OK;
+    }
+    if (!tg || !sr || !out) return fail(FMMA_ERR_INVALID, "null pointer");
     if (!pl.ops.p2p || pl.ops.dim != pl.dim || pl.ops.dtype != pl.dtype)
         return fail(FMMA_ERR_INVALID, "kernel launchers do not match the plan's DIM/TYPE");
     if (pl.timing && !pl.events_ready) {
         for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
         pl.events_ready = true;
     }
-    if (!is_fmm) pl.mark(0);
+    const bool staged = is_fmm && op == Op::Solve;  // the fmm solve records its own stage marks
+    if (!staged) pl.mark(0);
     int rc;
     if (pl.dtype == FMMA_F64) {
-        if (pl.dim == 1) rc = solve_typed<double, 1>(pl, nt, tg, w, ns, sr, ans, is_fmm);
-        else if (pl.dim == 2) rc = solve_typed<double, 2>(pl, nt, tg, w, ns, sr, ans, is_fmm);
-        else rc = solve_typed<double, 3>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        if (pl.dim == 1) rc = op_typed<double, 1>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
+        else if (pl.dim == 2) rc = op_typed<double, 2>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
+        else rc = op_typed<double, 3>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
     } else {
-        if (pl.dim == 1) rc = solve_typed<float, 1>(pl, nt, tg, w, ns, sr, ans, is_fmm);
-        else if (pl.dim == 2) rc = solve_typed<float, 2>(pl, nt, tg, w, ns, sr, ans, is_fmm);
-        else rc = solve_typed<float, 3>(pl, nt, tg, w, ns, sr, ans, is_fmm);
+        if (pl.dim == 1) rc = op_typed<float, 1>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
+        else if (pl.dim == 2) rc = op_typed<float, 2>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
+        else rc = op_typed<float, 3>(pl, op, nt, tg, w, ns, sr, out, is_fmm, reuse);
     }
     if (rc) return rc;
-    if (!is_fmm) pl.mark(kStages);
+    if (!staged) pl.mark(kStages);
     FMMA_HIP(hipStreamSynchronize(pl.stream));
     if (pl.timing && pl.events_ready) {
         float total = 0.f;
         FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
-        if (is_fmm) {
+        if (staged) {
             for (int i = 0; i < kStages; ++i) {
                 float ms = 0.f;
                 FMMA_HIP(hipEventElapsedTime(&ms, pl.ev[i], pl.ev[i + 1]));
                 pl.time_log[kStageNames[i]] += ms;
             }
         }
-        pl.time_log["total time(" + s + ")"] = total;
+        const char *what = op == Op::Solve ? "" : op == Op::Gradient ? " gradient" : " transpose";
+        pl.time_log["total time(" + s + what + ")"] = total;
     }
     return FMMA_OK;
+}
+
+static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans)
+{
+    return run_op(pl, Op::Solve, nt, tg, w, ns, sr, ans, false);
 }
 
 }  // namespace lib
@@ -527,6 +658,7 @@ int fmma_plan_set_poly_ord(fmma_plan plan, int poly_ord)
     if (poly_ord < 0 || poly_ord + 1 > fmma::lib::kMaxNodes)
         return fmma::lib::fail(FMMA_ERR_INVALID, "poly_ord must be in [0, 15]");
     plan->pl.poly_ord = poly_ord;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -534,6 +666,7 @@ int fmma_plan_set_depth(fmma_plan plan, int depth)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.depth = depth;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -548,6 +681,7 @@ int fmma_plan_set_trans_sym_flag(fmma_plan plan, int flag)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.trans_sym = flag;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -576,6 +710,7 @@ int fmma_plan_set_kernel(fmma_plan plan, int kernel_id, const double *params, in
     if (rc) return fmma::lib::fail(rc, "unknown kernel id " + std::to_string(kernel_id));
     plan->pl.ops = o;
     plan->pl.kernel_id = kernel_id;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -587,6 +722,7 @@ int fmma_plan_set_kernel_ops(fmma_plan plan, const fmma_kernel_ops *ops)
         return fmma::lib::fail(FMMA_ERR_INVALID, "kernel table incomplete or of another DIM/TYPE");
     plan->pl.ops = *ops;
     plan->pl.kernel_id = FMMA_KERNEL_CUSTOM;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -596,6 +732,7 @@ int fmma_plan_set_m2l_precision(fmma_plan plan, int precision)
     if (precision < FMMA_M2L_NATIVE || precision > FMMA_M2L_BF16X3)
         return fmma::lib::fail(FMMA_ERR_INVALID, "unknown M2L precision");
     plan->pl.precision = precision;
+    ++plan->pl.gen;
     return FMMA_OK;
 }
 
@@ -627,29 +764,82 @@ int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *w
     FMMA_GUARD(return fmma::lib::solve_device(plan->pl, n_t, targets, weights, n_s, sources, ans));
 }
 
-int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
-                         const void *sources, void *ans)
+// Host-pointer form of run_op: stages the inputs through device buffers.
+// vec is the weights [n_s] (solve, gradient) or the cotangent [n_t]
+// (transpose); out is ans [n_t] (read too: a shallow fmm adds onto it),
+// the gradient [n_t][dim] or the transpose [n_s].
+static int run_op_host(Plan &pl, fmma::lib::Op op, long n_t, const void *targets, const void *vec, long n_s,
+                       const void *sources, void *out, bool reuse)
 {
-    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
-    Plan &pl = plan->pl;
+    using fmma::lib::Op;
     if (n_t < 0 || n_s < 0) return fmma::lib::fail(FMMA_ERR_INVALID, "negative size");
     const size_t ts = pl.tsize(), D = pl.dim;
-    if (n_t == 0 || n_s == 0) return fmma::lib::solve_device(pl, n_t, targets, weights, n_s, sources, ans);
-    if (!targets || !sources || !ans) return fmma::lib::fail(FMMA_ERR_INVALID, "null pointer");
+    const long n_vec = op == Op::Transpose ? n_t : n_s;
+    const long n_out = op == Op::Solve ? n_t : op == Op::Gradient ? n_t * (long)D : n_s;
+    if (n_t == 0 || n_s == 0) {
+        if (op == Op::Solve) return fmma::lib::run_op(pl, op, n_t, targets, vec, n_s, sources, out, reuse);
+        if (n_out > 0) {
+            if (!out) return fmma::lib::fail(FMMA_ERR_INVALID, "null pointer");
+            std::memset(out, 0, n_out * ts);
+        }
+        return FMMA_OK;
+    }
+    if (!targets || !sources || !out) return fmma::lib::fail(FMMA_ERR_INVALID, "null pointer");
     int rc;
     hipStream_t st = pl.stream;
     const bool shared = targets == sources && n_t == n_s;
     if ((rc = fmma::lib::upload(pl.h_s, sources, n_s * D * ts, st))) return rc;
     if (!shared && (rc = fmma::lib::upload(pl.h_t, targets, n_t * D * ts, st))) return rc;
-    if (weights && (rc = fmma::lib::upload(pl.h_w, weights, n_s * ts, st))) return rc;
-    if ((rc = fmma::lib::upload(pl.h_ans, ans, n_t * ts, st))) return rc;
+    if (vec && (rc = fmma::lib::upload(pl.h_w, vec, n_vec * ts, st))) return rc;
+    if (op == Op::Solve) {
+        if ((rc = fmma::lib::upload(pl.h_ans, out, n_out * ts, st))) return rc;
+    } else if ((rc = pl.h_ans.ensure(n_out * ts))) {
+        return rc;
+    }
     const void *dt = shared ? pl.h_s.p : pl.h_t.p;
-    FMMA_GUARD(rc = fmma::lib::solve_device(pl, n_t, dt, weights ? pl.h_w.p : nullptr, n_s, pl.h_s.p, pl.h_ans.p));
+    FMMA_GUARD(rc = fmma::lib::run_op(pl, op, n_t, dt, vec ? pl.h_w.p : nullptr, n_s, pl.h_s.p, pl.h_ans.p, reuse));
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(ans, pl.h_ans.p, n_t * ts, hipMemcpyDeviceToHost, st);
+    hipError_t e = hipMemcpyAsync(out, pl.h_ans.p, n_out * ts, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return fmma::lib::hip_fail(e, "copy ans to host");
+    if (e != hipSuccess) return fmma::lib::hip_fail(e, "copy the result to the host");
     return FMMA_OK;
+}
+
+int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                         const void *sources, void *ans)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    return run_op_host(plan->pl, fmma::lib::Op::Solve, n_t, targets, weights, n_s, sources, ans, false);
+}
+
+int fmma_plan_gradient(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                       const void *sources, void *grad, int reuse)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    FMMA_GUARD(return fmma::lib::run_op(plan->pl, fmma::lib::Op::Gradient, n_t, targets, weights, n_s, sources, grad,
+                                        reuse != 0));
+}
+
+int fmma_plan_gradient_host(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                            const void *sources, void *grad)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    return run_op_host(plan->pl, fmma::lib::Op::Gradient, n_t, targets, weights, n_s, sources, grad, false);
+}
+
+int fmma_plan_solve_transpose(fmma_plan plan, long n_t, const void *targets, const void *cot, long n_s,
+                              const void *sources, void *out)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    FMMA_GUARD(return fmma::lib::run_op(plan->pl, fmma::lib::Op::Transpose, n_t, targets, cot, n_s, sources, out,
+                                        false));
+}
+
+int fmma_plan_solve_transpose_host(fmma_plan plan, long n_t, const void *targets, const void *cot, long n_s,
+                                   const void *sources, void *out)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    return run_op_host(plan->pl, fmma::lib::Op::Transpose, n_t, targets, cot, n_s, sources, out, false);
 }
 
 int fmma_plan_time_log(fmma_plan plan, const char **names, double *ms, int max)

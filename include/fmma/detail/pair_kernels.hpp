@@ -3,6 +3,7 @@
 // Templated on a device functor K with
 //     static constexpr bool translation_invariant;
 //     template <typename T, int D> __device__ T operator()(const T *x, const T *y) const;
+// and optionally far(), grad(), grad_far() and `static constexpr int symmetry`.
 // The library instantiates them for its built-in kernels; a caller compiled
 // with hipcc instantiates them for its own functor (include/fmma/fmma_device.hpp)
 // and hands the launchers to a plan through fmma_kernel_ops.
@@ -12,6 +13,8 @@
 //   exact        src/fmma.cpp:79-94
 //   M2L operator src/fmm_core.cpp:388-418 (translation-symmetric branch)
 //   M2L general  src/fmm_core.cpp:456-492 (kernel evaluated per box pair)
+// and, new, the near field and all-pairs forms of the gradient with respect
+// to the target coordinates.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -53,6 +56,42 @@ __device__ __forceinline__ T pair_eval(const K &k, const T *x, const T *y)
     else
         return k.template operator()<T, D>(x, y);
 }
+
+// A functor may provide grad(x, y, g): g[d] = d/dx_d f(x, y) (the near
+// field of the gradient), and grad_far() without its r = 0 case.
+template <class K, typename T, int D, typename = void>
+struct has_grad : std::false_type {};
+template <class K, typename T, int D>
+struct has_grad<K, T, D,
+                std::void_t<decltype(std::declval<const K &>().template grad<T, D>(
+                    (const T *)nullptr, (const T *)nullptr, (T *)nullptr))>> : std::true_type {};
+template <class K, typename T, int D, typename = void>
+struct has_grad_far : std::false_type {};
+template <class K, typename T, int D>
+struct has_grad_far<K, T, D,
+                    std::void_t<decltype(std::declval<const K &>().template grad_far<T, D>(
+                        (const T *)nullptr, (const T *)nullptr, (T *)nullptr))>> : std::true_type {};
+
+template <bool FAR, typename T, int D, class K>
+__device__ __forceinline__ void pair_grad(const K &k, const T *x, const T *y, T *g)
+{
+    if constexpr (FAR && has_grad_far<K, T, D>::value)
+        k.template grad_far<T, D>(x, y, g);
+    else
+        k.template grad<T, D>(x, y, g);
+}
+
+// f^T(x, y) = f(y, x): the kernel of the transposed solve.
+template <class K>
+struct Transposed {
+    K k;
+    static constexpr bool translation_invariant = K::translation_invariant;
+    template <typename T, int D>
+    __device__ __forceinline__ T operator()(const T *x, const T *y) const
+    {
+        return k.template operator()<T, D>(y, x);
+    }
+};
 
 // ----------------------------------------------------------------- P2P
 //
@@ -506,6 +545,221 @@ int launch_p2p(const fmma_p2p_args *a, const K &k, hipStream_t s)
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }
 
+// ------------------------------------------------------ P2P gradient
+//
+// The near field of the gradient with respect to the target coordinates,
+// laid out as p2p_kernel (one wave per target box, slots x slices, sources
+// of the 3^D neighbourhood staged through LDS) with D sums per target.
+constexpr int kP2PGradTargets = 8;
+
+template <int TPL, bool FAR, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_grad_range(const T *src, unsigned j0, unsigned j1, unsigned q, unsigned nslice,
+                                               const T (&x)[NT][D], T (&acc)[NT][D], const K &kern)
+{
+    constexpr int RW = p2p_rec<D>::W;
+    const unsigned r0 = j0 % nslice;
+    for (unsigned j = j0 + (q >= r0 ? q - r0 : q + nslice - r0); j < j1; j += nslice) {
+        const T *r = src + j * RW;
+        T y[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) y[d] = r[d];
+        const T wj = r[D];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            T g[D];
+            pair_grad<FAR, T, D>(kern, x[k], y, g);
+#pragma unroll
+            for (int d = 0; d < D; ++d) acc[k][d] += wj * g[d];
+        }
+    }
+}
+
+template <int TPL, typename T, int D, int NT, class K>
+__device__ __forceinline__ void p2p_grad_block(const T *src, unsigned cnt, unsigned nself, unsigned q,
+                                               unsigned nslice, const T (&x)[NT][D], T (&acc)[NT][D], const K &kern)
+{
+    const unsigned m = nself < cnt ? nself : cnt;
+    if (m > 0) p2p_grad_range<TPL, false>(src, 0, m, q, nslice, x, acc, kern);
+    p2p_grad_range<TPL, true>(src, m, cnt, q, nslice, x, acc, kern);
+}
+
+template <typename T, int D, class K>
+__global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K kern)
+{
+    constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
+    constexpr int RW = p2p_rec<D>::W;
+    constexpr int MT = kP2PGradTargets;
+    __shared__ __align__(16) T src[kP2PBlock * RW];
+    __shared__ unsigned nb_first[NNB], nb_pre[NNB + 1];
+
+    const unsigned box = blockIdx.x;
+    const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
+    if (t0 == t1) return;
+    const unsigned lane = threadIdx.x;
+    const T *tp = static_cast<const T *>(a.t_pos);
+    const T *sp = static_cast<const T *>(a.s_pos);
+    const T *sw = static_cast<const T *>(a.s_w);
+    const T *base = static_cast<const T *>(a.base);
+    T *grad = static_cast<T *>(a.grad);
+    const long nt = a.n_t, ns = a.n_s;
+
+    // neighbour ranges, the box itself first (as p2p_kernel)
+    unsigned cnt_nb = 0;
+    if (lane < (unsigned)NNB) {
+        int bc[D], m[D];
+        morton_decode<D>(box, bc);
+        const int NL = 1 << a.leaf_level;
+        bool ok = true;
+        constexpr int centre = (NNB - 1) / 2;
+        int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            m[d] = bc[d] + (k % 3) - 1;
+            k /= 3;
+            ok = ok && m[d] >= 0 && m[d] < NL;
+        }
+        unsigned first = 0;
+        if (ok) {
+            const unsigned mm = morton_encode<D>(m);
+            first = a.s_start[mm];
+            cnt_nb = a.s_start[mm + 1] - first;
+        }
+        nb_first[lane] = first;
+    }
+    unsigned incl = cnt_nb;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += y;
+    }
+    if (lane < (unsigned)NNB) nb_pre[lane + 1] = incl;
+    if (lane == 0) nb_pre[0] = 0;
+    __syncthreads();
+    const unsigned total = nb_pre[NNB], nself = nb_pre[1];
+
+    const unsigned ntb = t1 - t0;
+    unsigned nslot = 4;
+    while (nslot < 64 && nslot * MT < ntb) nslot <<= 1;
+    const unsigned nslice = 64 / nslot, s = lane % nslot, q = lane / nslot;
+
+    for (unsigned pass0 = t0; pass0 < t1; pass0 += nslot * MT) {
+        const unsigned npass = t1 - pass0 < nslot * MT ? t1 - pass0 : nslot * MT;
+        const int tpl = (int)((npass + nslot - 1) / nslot);
+        T x[MT][D], acc[MT][D];
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            const unsigned i = pass0 + s + nslot * k;
+            const unsigned ic = i < t1 ? i : t1 - 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                acc[k][d] = T(0);
+                x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
+            }
+        }
+        for (unsigned b0 = 0; b0 < total; b0 += kP2PBlock) {
+            const unsigned cnt = total - b0 < (unsigned)kP2PBlock ? total - b0 : (unsigned)kP2PBlock;
+            __syncthreads();
+            int b = 0;
+            for (unsigned e = lane; e < cnt; e += 64) {
+                const unsigned p = b0 + e;
+                while (nb_pre[b + 1] <= p) ++b;
+                const long j = (long)nb_first[b] + (p - nb_pre[b]);
+                T *r = src + e * RW;
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = sp[d * ns + j];
+                r[D] = sw[j];
+            }
+            __syncthreads();
+            const unsigned ns_blk = nself > b0 ? nself - b0 : 0u;
+            switch (tpl) {
+            case 1: p2p_grad_block<1>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 2: p2p_grad_block<2>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 3: p2p_grad_block<3>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 4: p2p_grad_block<4>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 5: p2p_grad_block<5>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 6: p2p_grad_block<6>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            case 7: p2p_grad_block<7>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            default: p2p_grad_block<8>(src, cnt, ns_blk, q, nslice, x, acc, kern); break;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            if (k < tpl) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    T v = acc[k][d];
+                    for (unsigned o = nslot; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
+                    acc[k][d] = v;
+                }
+            }
+        }
+        if (q == 0) {
+#pragma unroll
+            for (int k = 0; k < MT; ++k) {
+                const unsigned i = pass0 + s + nslot * k;
+                if (k >= tpl || i >= t1) continue;
+                const long oi = a.t_perm[i];
+#pragma unroll
+                for (int d = 0; d < D; ++d) grad[oi * D + d] = base ? base[d * nt + i] + acc[k][d] : acc[k][d];
+            }
+        }
+    }
+}
+
+template <typename T, int D, class K>
+int launch_p2p_grad(const fmma_p2p_grad_args *a, const K &k, hipStream_t s)
+{
+    if constexpr (has_grad<K, T, D>::value) {
+        if (a->n_target_boxes <= 0) return FMMA_OK;
+        hipLaunchKernelGGL((p2p_grad_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+        return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
+    } else {
+        return FMMA_ERR_UNSUPPORTED;
+    }
+}
+
+// All-pairs gradient: one thread per target, sources in ascending order.
+template <typename T, int D, class K>
+__global__ __launch_bounds__(256) void exact_grad_kernel(fmma_exact_grad_args a, K kern)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n_t) return;
+    const T *tg = static_cast<const T *>(a.targets);
+    const T *sr = static_cast<const T *>(a.sources);
+    const T *w = static_cast<const T *>(a.weights);
+    T x[D], acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        x[d] = tg[i * D + d];
+        acc[d] = T(0);
+    }
+    for (long j = 0; j < a.n_s; ++j) {
+        T y[D], g[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) y[d] = sr[j * D + d];
+        kern.template grad<T, D>(x, y, g);
+        const T wj = w ? w[j] : T(1);
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] += wj * g[d];
+    }
+    T *out = static_cast<T *>(a.grad);
+#pragma unroll
+    for (int d = 0; d < D; ++d) out[i * D + d] = acc[d];
+}
+
+template <typename T, int D, class K>
+int launch_exact_grad(const fmma_exact_grad_args *a, const K &k, hipStream_t s)
+{
+    if constexpr (has_grad<K, T, D>::value) {
+        if (a->n_t <= 0) return FMMA_OK;
+        hipLaunchKernelGGL((exact_grad_kernel<T, D, K>), dim3((unsigned)((a->n_t + 255) / 256)), dim3(256), 0, s, *a,
+                           k);
+        return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
+    } else {
+        return FMMA_ERR_UNSUPPORTED;
+    }
+}
+
 // --------------------------------------------------------------- exact
 //
 // All pairs.  One thread per target; sources are wave-uniform (scalar loads).
@@ -705,6 +959,14 @@ struct ops_for {
     {
         return launch_m2l_general<T, D, K>(a, *static_cast<const K *>(ctx), (hipStream_t)s);
     }
+    static int p2p_grad(const fmma_p2p_grad_args *a, void *ctx, void *s)
+    {
+        return launch_p2p_grad<T, D, K>(a, *static_cast<const K *>(ctx), (hipStream_t)s);
+    }
+    static int exact_grad(const fmma_exact_grad_args *a, void *ctx, void *s)
+    {
+        return launch_exact_grad<T, D, K>(a, *static_cast<const K *>(ctx), (hipStream_t)s);
+    }
     static fmma_kernel_ops make(const K *ctx)
     {
         fmma_kernel_ops o;
@@ -717,6 +979,9 @@ struct ops_for {
         o.exact = &exact;
         o.m2l_op = &m2l_op;
         o.m2l_general = &m2l_general;
+        o.p2p_grad = has_grad<K, T, D>::value ? &p2p_grad : nullptr;
+        o.exact_grad = has_grad<K, T, D>::value ? &exact_grad : nullptr;
+        o.transposed = nullptr;
         return o;
     }
 };

@@ -14,7 +14,8 @@
 //    built-in kernels (set_kernel); a host-only std::function cannot run on
 //    the GPU and is rejected at compile time.
 //  * "tree" and "nrnmm" throw std::runtime_error (not built for the GPU).
-//  * New setters: set_kernel, set_m2l_precision; new getter get_time_log.
+//  * New setters: set_kernel, set_m2l_precision; new getter get_time_log;
+//    new differentiation entry points gradient and solve_transpose.
 // Misuse the reference ends with exit(EXIT_FAILURE) ends the same way here:
 // an unknown solver type (src/fmma.cpp:183-186) and a source_weight size
 // that differs from the source size (src/fmm.cpp:32-35).
@@ -66,6 +67,10 @@ class FMMA {
     // f(x - y) (include/fmma/fmma.hpp:65-72), a two-argument f as f(x, y).
     template <class F>
     void set_fn(F f);
+    // f with its gradient grad_f(x, y) -> std::array<TYPE, DIM> = grad_x f
+    // (or grad_f(x - y) for a one-argument f), for gradient().
+    template <class F, class G>
+    void set_fn(F f, G grad_f);
     template <class F>
     void set_fn1(F f) { set_fn(f); }
     template <class F>
@@ -136,9 +141,46 @@ class FMMA {
     FMMA_SOLVER_PAIR(fmm)
 #undef FMMA_SOLVER_PAIR
 
+    // ---- differentiation (new; the reference has none)
+    // grad[i][d] = d ans_i / d x_{i,d} of the current solver's result
+    // (fmm: the derivative of the approximation; exact: of the direct sum).
+    // Needs a kernel with a gradient: a built-in, or set_fn(f, grad_f).
+    void gradient(const std::vector<std::array<TYPE, DIM>> &target, const std::vector<TYPE> &source_weight,
+                  const std::vector<std::array<TYPE, DIM>> &source, std::vector<std::array<TYPE, DIM>> &grad)
+    {
+        grad.resize(target.size());
+        check_sizes(target, &source_weight, source);
+        check(fmma_plan_gradient_host(plan_, (long)target.size(), ptr(target), source_weight.data(),
+                                      (long)source.size(), ptr(source), grad.empty() ? nullptr : grad.data()->data()));
+    }
+    // out[j] = sum_i cot[i] f(x_i, y_j): the transpose of the solve in the
+    // weights, on the tree and Depth that solve(target, ., source) uses.
+    void solve_transpose(const std::vector<std::array<TYPE, DIM>> &target, const std::vector<TYPE> &cot,
+                         const std::vector<std::array<TYPE, DIM>> &source, std::vector<TYPE> &out)
+    {
+        out.resize(source.size());
+        if (cot.size() != target.size()) throw std::runtime_error("fmma: cot size != target size");
+        check(fmma_plan_solve_transpose_host(plan_, (long)target.size(), ptr(target), cot.data(),
+                                             (long)source.size(), ptr(source), out.data()));
+    }
+
     fmma_plan plan() const { return plan_; }
 
   private:
+    static const void *ptr(const std::vector<std::array<TYPE, DIM>> &v)
+    {
+        return v.empty() ? nullptr : v.data()->data();
+    }
+    static void check_sizes(const std::vector<std::array<TYPE, DIM>> &target, const std::vector<TYPE> *source_weight,
+                            const std::vector<std::array<TYPE, DIM>> &source)
+    {
+        if (source_weight && !target.empty() && !source.empty() && source_weight->size() != source.size()) {
+            fprintf(stderr, "%s:%d ERROR : source_weight size (%zu) != source size (%zu)\n", __FILE__, __LINE__,
+                    source_weight->size(), source.size());
+            exit(EXIT_FAILURE);
+        }
+    }
+
     static void check(int rc)
     {
         if (rc != FMMA_OK) throw std::runtime_error(std::string("fmma: ") + fmma_last_error());
@@ -149,11 +191,7 @@ class FMMA {
              std::vector<TYPE> &ans)
     {
         ans.resize(target.size());
-        if (source_weight && !target.empty() && !source.empty() && source_weight->size() != source.size()) {
-            fprintf(stderr, "%s:%d ERROR : source_weight size (%zu) != source size (%zu)\n", __FILE__, __LINE__,
-                    source_weight->size(), source.size());
-            exit(EXIT_FAILURE);
-        }
+        check_sizes(target, source_weight, source);
         const bool swap = solver != solver_type_;
         if (swap) check(fmma_plan_set_solver_type(plan_, solver.c_str()));
         // std::array<TYPE, DIM> elements are contiguous: the vectors are the
@@ -208,6 +246,14 @@ void FMMA<TYPE, DIM>::set_fn(F)
 {
     static_assert(sizeof(F) == 0,
                   "FMMA::set_fn takes a __host__ __device__ functor and must be compiled with hipcc; "
+                  "use set_kernel(FMMA_KERNEL_*) for the built-in kernels");
+}
+template <typename TYPE, std::size_t DIM>
+template <class F, class G>
+void FMMA<TYPE, DIM>::set_fn(F, G)
+{
+    static_assert(sizeof(F) == 0,
+                  "FMMA::set_fn takes __host__ __device__ functors and must be compiled with hipcc; "
                   "use set_kernel(FMMA_KERNEL_*) for the built-in kernels");
 }
 }  // namespace fmma

@@ -103,6 +103,35 @@ typedef struct {
     int split;                 /* number of source slices summed in fixed order */
 } fmma_exact_args;
 
+/* Near field of the gradient with respect to the target coordinates:
+ * grad[i][d] = base[d][i] + sum_j w_j d/dx_d f(x_i, y_j) over the sources of
+ * the 3^dim neighbour boxes, for the fmm solver's gradient.  Same tree
+ * arrays as fmma_p2p_args. */
+typedef struct {
+    int dim, dtype;
+    long n_target_boxes;
+    int leaf_level;
+    const unsigned *t_start;   /* [n_boxes+1] */
+    const unsigned *s_start;   /* [n_boxes+1] */
+    const void *t_pos;         /* [dim][n_t] sorted */
+    const void *s_pos;         /* [dim][n_s] sorted */
+    const void *s_w;           /* [n_s] sorted */
+    long n_t, n_s;
+    const void *base;          /* [dim][n_t] far-field gradient, sorted order, or NULL */
+    const unsigned *t_perm;    /* [n_t] */
+    void *grad;                /* [n_t][dim] caller's output, original order, overwritten */
+} fmma_p2p_grad_args;
+
+/* All-pairs gradient: grad[i][d] = sum_j w_j d/dx_d f(x_i, y_j). */
+typedef struct {
+    int dim, dtype;
+    long n_t, n_s;
+    const void *targets;       /* [n_t][dim] */
+    const void *weights;       /* [n_s] or NULL for all-ones */
+    const void *sources;       /* [n_s][dim] */
+    void *grad;                /* [n_t][dim] */
+} fmma_exact_grad_args;
+
 /* The bounding cube lives on the device (it is reduced there); the box side
  * at a level is len = Len / (TYPE)(1 << level), as in src/fmm_core.cpp:376. */
 typedef struct {
@@ -131,7 +160,7 @@ typedef struct {
 /* Each launcher gets the table's `ctx` back (the functor's state, or the
  * built-in kernel's parameters) and the plan's hipStream_t; it returns an
  * FMMA_* status and must not synchronise. */
-typedef struct {
+typedef struct fmma_kernel_ops_s {
     int dim, dtype;
     int translation_invariant; /* 1: f(x,y) = g(x-y); M2L may use per-offset operators */
     int symmetry;              /* +1: f(y,x) = f(x,y), -1: f(y,x) = -f(x,y), 0: neither or unknown;
@@ -141,6 +170,12 @@ typedef struct {
     int (*exact)(const fmma_exact_args *, void *ctx, void *stream);
     int (*m2l_op)(const fmma_m2l_op_args *, void *ctx, void *stream);
     int (*m2l_general)(const fmma_m2l_general_args *, void *ctx, void *stream);
+    /* Optional (NULL when f has no gradient): grad_x f for fmma_plan_gradient. */
+    int (*p2p_grad)(const fmma_p2p_grad_args *, void *ctx, void *stream);
+    int (*exact_grad)(const fmma_exact_grad_args *, void *ctx, void *stream);
+    /* The table of f^T(x,y) = f(y,x), for fmma_plan_solve_transpose when
+     * symmetry == 0 (NULL otherwise: f^T is then symmetry * f). */
+    const struct fmma_kernel_ops_s *transposed;
 } fmma_kernel_ops;
 
 /* ------------------------------------------------------------ plan life */
@@ -216,6 +251,33 @@ int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *w
  * memory (the copies are outside the per-stage time log). */
 int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights,
                          long n_s, const void *sources, void *ans);
+
+/* New (differentiation): the gradient of fmma_plan_solve's result with
+ * respect to the target coordinates, grad[i][d] = d ans_i / d x_{i,d}, for
+ * the solvers "fmm" (the derivative of the fmm approximation: the L2P
+ * interpolant differentiated through the Chebyshev basis of
+ * src/fmm_core.cpp:602-612, plus sum_j w_j grad_x f over the neighbour boxes)
+ * and "exact" / "exact_matvec" (all pairs).  DEVICE pointers; grad is
+ * [n_t][dim], overwritten.  reuse = 1 takes the local expansions of this
+ * plan's previous fmm solve when it had the same pointers, counts and
+ * settings; otherwise, and with reuse = 0, the fmm passes run again.  Needs a
+ * kernel with a gradient (every built-in, or fmma_kernel_ops.p2p_grad). */
+int fmma_plan_gradient(fmma_plan plan, long n_t, const void *targets, const void *weights,
+                       long n_s, const void *sources, void *grad, int reuse);
+int fmma_plan_gradient_host(fmma_plan plan, long n_t, const void *targets, const void *weights,
+                            long n_s, const void *sources, void *grad);
+
+/* New (differentiation): the transpose in the weights, out[j] = sum_i cot[i]
+ * f(x_i, y_j) for j < n_s, i.e. d(sum_i cot_i ans_i)/d w_j.  For "fmm" it is
+ * the fmm solver run with targets and sources swapped and f transposed, on
+ * the tree and Depth that fmma_plan_solve(targets, ., sources) uses (Depth
+ * from n_t when still automatic), so it is the exact adjoint of that solve's
+ * operator up to rounding.  DEVICE pointers: targets [n_t][dim], cot [n_t],
+ * sources [n_s][dim], out [n_s], overwritten. */
+int fmma_plan_solve_transpose(fmma_plan plan, long n_t, const void *targets, const void *cot,
+                              long n_s, const void *sources, void *out);
+int fmma_plan_solve_transpose_host(fmma_plan plan, long n_t, const void *targets, const void *cot,
+                                   long n_s, const void *sources, void *out);
 
 /* Per-stage milliseconds, accumulated over solves like the reference's
  * time_log (include/fmma/fmma.hpp:168, src/fmm.cpp:57-141); the total is

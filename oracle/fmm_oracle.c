@@ -24,6 +24,12 @@
  *   P2P and stage order              src/fmm.cpp:15-144
  *   exact                            src/fmma.cpp:79-94
  *
+ * Differentiation (no counterpart in the reference; the gradient is that of
+ * the approximation above): oracle_kernel_grad (each kernel's formula
+ * differentiated), oracle_cheb_dS (the L2P basis of src/fmm_core.cpp:602-612
+ * differentiated), oracle_fmm_grad, oracle_exact_grad.  Pinned by central
+ * differences of oracle_kernel, oracle_cheb_S, oracle_exact and oracle_fmm.
+ *
  * Pinning (tests/test_oracle.py): Chebyshev closed forms T_0..T_3, discrete
  * orthogonality S_n(c_i,c_j)=delta_ij, interpolation exactness for
  * polynomials of degree < n, M2M(P2M(child)) == P2M(parent), the L2L/M2M
@@ -97,6 +103,56 @@ double oracle_kernel(int id, const double *prm, int dim, const double *x, const 
     }
 }
 
+/* grad_x f(x, y) of each kernel, written from its formula (differentiation
+ * is new; the reference has none).  Kernels with an r = 0 special case take
+ * the gradient as 0 there. */
+void oracle_kernel_grad(int id, const double *prm, int dim, const double *x, const double *y, double *g)
+{
+    double t[OR_MAXD], r2 = 0.0;
+    for (int d = 0; d < dim; ++d) { t[d] = x[d] - y[d]; r2 += t[d] * t[d]; }
+    double c = 0.0;
+    switch (id) {
+    case ORACLE_K_INV_R:
+    case ORACLE_K_INV_R_REG:
+        c = r2 == 0.0 ? 0.0 : -1.0 / (r2 * sqrt(r2));
+        break;
+    case ORACLE_K_INV_DIFF:
+        for (int d = 0; d < dim; ++d) g[d] = 0.0;
+        g[0] = -1.0 / (t[0] * t[0]);
+        return;
+    case ORACLE_K_LOG_R:
+        c = r2 == 0.0 ? 0.0 : 1.0 / r2;
+        break;
+    case ORACLE_K_GAUSS: {
+        double h = prm[0];
+        c = -exp(-r2 / (2.0 * h * h)) / (h * h);
+        break;
+    }
+    case ORACLE_K_YUKAWA: {
+        if (r2 == 0.0) { c = 0.0; break; }
+        double r = sqrt(r2), k = prm[0];
+        c = -(1.0 + k * r) * exp(-k * r) / (r2 * r);
+        break;
+    }
+    case ORACLE_K_SUM_R2: {
+        double s = 0.0;
+        for (int d = 0; d < dim; ++d) s += t[d];
+        for (int d = 0; d < dim; ++d) g[d] = 1.0 / r2 - 2.0 * s * t[d] / (r2 * r2);
+        return;
+    }
+    case ORACLE_K_SUM2_R2: {
+        double s2 = 0.0;
+        for (int d = 0; d < dim; ++d) { double u = x[d] + y[d]; s2 += u * u; }
+        for (int d = 0; d < dim; ++d) g[d] = 2.0 * (x[d] + y[d]) / r2 - 2.0 * s2 * t[d] / (r2 * r2);
+        return;
+    }
+    default:
+        fprintf(stderr, "oracle: unknown kernel id %d\n", id);
+        abort();
+    }
+    for (int d = 0; d < dim; ++d) g[d] = c * t[d];
+}
+
 /* ---------------------------------------------------------------- Chebyshev */
 
 /* T_j(x) = cos(j acos x), defined on [-1,1] only (src/math.cpp:14-26). */
@@ -115,6 +171,21 @@ double oracle_cheb_S(int n, double x, double y)
 {
     double s = 1.0 / n;
     for (int j = 1; j < n; ++j) s += 2.0 / n * oracle_cheb_T(j, x) * oracle_cheb_T(j, y);
+    return s;
+}
+
+/* d/dx S_n(x, y) = (2/n) sum_{j=1}^{n-1} T_j'(x) T_j(y), T_j' = j U_{j-1} with
+ * U_0 = 1, U_1 = 2x, U_k = 2x U_{k-1} - U_{k-2}: the basis of L2P
+ * (src/fmm_core.cpp:602-612) differentiated in its first argument. */
+double oracle_cheb_dS(int n, double x, double y)
+{
+    double s = 0.0, um = 0.0, u = 1.0;  /* U_{j-2}, U_{j-1} */
+    for (int j = 1; j < n; ++j) {
+        s += 2.0 / n * (j * u) * oracle_cheb_T(j, y);
+        double un = 2.0 * x * u - um;
+        um = u;
+        u = un;
+    }
     return s;
 }
 
@@ -519,6 +590,22 @@ int oracle_exact(int dim, int kid, const double *prm, long nt, const double *t,
     return 0;
 }
 
+/* grad[i][d] = sum_j w_j d/dx_d f(x_i, y_j), sources in ascending order. */
+int oracle_exact_grad(int dim, int kid, const double *prm, long nt, const double *t,
+                      long ns, const double *w, const double *s, double *grad)
+{
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nt; ++i) {
+        double a[OR_MAXD] = {0.0, 0.0, 0.0}, g[OR_MAXD];
+        for (long j = 0; j < ns; ++j) {
+            oracle_kernel_grad(kid, prm, dim, t + i * dim, s + j * dim, g);
+            for (int d = 0; d < dim; ++d) a[d] += w[j] * g[d];
+        }
+        for (int d = 0; d < dim; ++d) grad[i * dim + d] = a[d];
+    }
+    return 0;
+}
+
 int oracle_auto_depth(long nt, int dim)
 {
     int depth = (int)(log((double)nt) / dim) + 1;
@@ -530,9 +617,31 @@ int oracle_auto_depth(long nt, int dim)
  * into, as in the reference.  When Wm_out/Wl_out are given they receive the
  * multipole/local coefficients of levels 0..depth-1 packed one after another
  * (row-major box ids; levels never built stay zero). */
+static int fmm_run(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
+                   long nt, const double *t, long ns, const double *w, const double *s,
+                   double *ans, double *grad, double *Wm_out, double *Wl_out);
+
 int oracle_fmm_levels(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
                       long nt, const double *t, long ns, const double *w, const double *s,
                       double *ans, double *Wm_out, double *Wl_out)
+{
+    return fmm_run(dim, poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, ans, NULL, Wm_out, Wl_out);
+}
+
+/* The gradient of the fmm result with respect to the target coordinates:
+ * the L2P interpolant differentiated (d r_d / d x_d = 2 / len_leaf) plus
+ * sum_j w_j grad_x f(x_i, y_j) over the neighbour boxes.  grad is [nt][dim],
+ * overwritten (zero for an empty sum). */
+int oracle_fmm_grad(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
+                    long nt, const double *t, long ns, const double *w, const double *s, double *grad)
+{
+    for (long i = 0; i < nt * dim; ++i) grad[i] = 0.0;
+    return fmm_run(dim, poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, NULL, grad, NULL, NULL);
+}
+
+static int fmm_run(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
+                   long nt, const double *t, long ns, const double *w, const double *s,
+                   double *ans, double *grad, double *Wm_out, double *Wl_out)
 {
     if (dim < 1 || dim > OR_MAXD || poly_ord < 0 || depth < 1) return -1;
     if (nt == 0 || ns == 0) return 0;
@@ -562,7 +671,50 @@ int oracle_fmm_levels(int dim, int poly_ord, int depth, int trans_sym, int kid, 
     for (int l = 2; l + 1 < D; ++l) l2l(&T, 2L << l, Wl[l], Wl[l + 1]);
 
     double len = T.Len / (double)T.NL;
-    if (D - 1 >= 2) {
+    if (grad && D - 1 >= 2) {
+        const double *W = Wl[D - 1];
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < nt; ++i) {
+            int b[OR_MAXD];
+            double rel[OR_MAXD];
+            for (int d = 0; d < dim; ++d) b[d] = bin1(t[i * dim + d], T.origin[d], len, T.NL, &rel[d]);
+            long bi = box_id(dim, b, T.NL);
+            for (int e = 0; e < dim; ++e) {
+                double a = 0.0;
+                for (int k = 0; k < P; ++k) {
+                    double v = 1.0;
+                    for (int d = 0; d < dim; ++d) {
+                        double c = T.c[T.digit[k * dim + d]];
+                        v *= d == e ? oracle_cheb_dS(T.n, rel[d], c) : oracle_cheb_S(T.n, rel[d], c);
+                    }
+                    a += W[bi * P + k] * v;
+                }
+                grad[i * dim + e] = a * (2.0 / len);
+            }
+        }
+    }
+    if (grad) {
+#pragma omp parallel
+        {
+            long nb[32];
+#pragma omp for schedule(dynamic, 64)
+            for (long i = 0; i < nt; ++i) {
+                int b[OR_MAXD];
+                for (int d = 0; d < dim; ++d) b[d] = bin1(t[i * dim + d], T.origin[d], len, T.NL, NULL);
+                int cnt = oracle_neighbour_list(dim, b, T.NL, nb);
+                double a[OR_MAXD], g[OR_MAXD];
+                for (int d = 0; d < dim; ++d) a[d] = grad[i * dim + d];
+                for (int q = 0; q < cnt; ++q)
+                    for (long j = lst_start[nb[q]]; j < lst_start[nb[q] + 1]; ++j) {
+                        long sj = lst[j];
+                        oracle_kernel_grad(kid, prm, dim, t + i * dim, s + sj * dim, g);
+                        for (int d = 0; d < dim; ++d) a[d] += w[sj] * g[d];
+                    }
+                for (int d = 0; d < dim; ++d) grad[i * dim + d] = a[d];
+            }
+        }
+    }
+    if (ans && D - 1 >= 2) {
         const double *W = Wl[D - 1];
 #pragma omp parallel for schedule(static)
         for (long i = 0; i < nt; ++i) {
@@ -581,11 +733,11 @@ int oracle_fmm_levels(int dim, int poly_ord, int depth, int trans_sym, int kid, 
         }
     }
 
-#pragma omp parallel
+#pragma omp parallel if (ans)
     {
         long nb[32];
 #pragma omp for schedule(dynamic, 64)
-        for (long i = 0; i < nt; ++i) {
+        for (long i = 0; i < (ans ? nt : 0); ++i) {
             int b[OR_MAXD];
             for (int d = 0; d < dim; ++d) b[d] = bin1(t[i * dim + d], T.origin[d], len, T.NL, NULL);
             int cnt = oracle_neighbour_list(dim, b, T.NL, nb);

@@ -21,7 +21,9 @@ enum {
 };
 
 double oracle_kernel(int id, const double *prm, int dim, const double *x, const double *y);
+void oracle_kernel_grad(int id, const double *prm, int dim, const double *x, const double *y, double *g);
 double oracle_cheb_T(int j, double x);
+double oracle_cheb_dS(int n, double x, double y);
 double oracle_cheb_S(int n, double x, double y);
 void oracle_nodes(int n, double *c);
 void oracle_bbox(int dim, long nt, const double *t, long ns, const double *s, double *origin, double *Len);
@@ -33,7 +35,11 @@ int oracle_interaction_list(int dim, const int *b, long N, long *out);
 int oracle_neighbour_list(int dim, const int *b, long N, long *out);
 int oracle_exact(int dim, int kid, const double *prm, long nt, const double *t,
                  long ns, const double *w, const double *s, double *ans);
+int oracle_exact_grad(int dim, int kid, const double *prm, long nt, const double *t,
+                      long ns, const double *w, const double *s, double *grad);
 int oracle_auto_depth(long nt, int dim);
+int oracle_fmm_grad(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
+                    long nt, const double *t, long ns, const double *w, const double *s, double *grad);
 int oracle_fmm(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
                long nt, const double *t, long ns, const double *w, const double *s, double *ans);
 int oracle_fmm_levels(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,

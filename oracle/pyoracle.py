@@ -58,6 +58,13 @@ def lib():
         L.oracle_p2m_level.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp,
                                        ctypes.c_long, _dp, _dp, _dp]
         L.oracle_m2m_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_kernel_grad.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_cheb_dS.restype = ctypes.c_double
+        L.oracle_cheb_dS.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.oracle_exact_grad.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _dp, ctypes.c_long,
+                                        _dp, _dp, _dp]
+        L.oracle_fmm_grad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      _dp, ctypes.c_long, _dp, ctypes.c_long, _dp, _dp, _dp]
         L.oracle_l2l_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
@@ -184,6 +191,56 @@ def fmm(targets, weights, sources, poly_ord, depth=-1, trans_sym=0, kid=0, param
     if levels:
         return out, split_levels(Wm, dim, P, depth), split_levels(Wl, dim, P, depth)
     return out
+
+
+def kernel_grad(kid, params, x, y):
+    """grad_x f(x, y) of one kernel at one pair of points."""
+    x, y, p = _c(x), _c(y), _prm(params)
+    g = np.zeros(x.shape[0])
+    lib().oracle_kernel_grad(kid, _d(p), x.shape[0], _d(x), _d(y), _d(g))
+    return g
+
+
+def cheb_dS(n, x, y):
+    return lib().oracle_cheb_dS(n, float(x), float(y))
+
+
+def exact_grad(targets, weights, sources, kid=0, params=()):
+    """grad[i, d] = sum_j w_j d/dx_d f(x_i, y_j)."""
+    t, w, s, p = _c(targets), _c(weights), _c(sources), _prm(params)
+    g = np.zeros(t.shape)
+    lib().oracle_exact_grad(t.shape[1], kid, _d(p), t.shape[0], _d(t), s.shape[0], _d(w), _d(s), _d(g))
+    return g
+
+
+def fmm_grad(targets, weights, sources, poly_ord, depth=-1, trans_sym=0, kid=0, params=()):
+    """Gradient of the oracle fmm result with respect to the target
+    coordinates: [N, dim]."""
+    t, w, s, p = _c(targets), _c(weights), _c(sources), _prm(params)
+    dim = t.shape[1]
+    if depth <= 0:
+        depth = auto_depth(t.shape[0], dim)
+    g = np.zeros(t.shape)
+    rc = lib().oracle_fmm_grad(dim, poly_ord, depth, trans_sym, kid, _d(p), t.shape[0], _d(t), s.shape[0], _d(w),
+                               _d(s), _d(g))
+    if rc != 0:
+        raise RuntimeError("oracle_fmm_grad failed (%d)" % rc)
+    return g
+
+
+SYMMETRY = {0: 1, 1: 1, 2: -1, 3: 1, 4: 1, 5: 1, 6: -1, 7: 1}
+
+
+def fmm_transpose(targets, cot, sources, poly_ord, depth=-1, trans_sym=0, kid=0, params=()):
+    """out[j] = sum_i cot_i f(x_i, y_j) by the oracle fmm with targets and
+    sources swapped (f^T = symmetry * f for every kernel id here) at the Depth
+    of the solve with (targets, sources)."""
+    t = _c(targets)
+    if depth <= 0:
+        depth = auto_depth(t.shape[0], t.shape[1])
+    s = _c(sources)
+    out = fmm(s, cot, t, poly_ord, depth, trans_sym, kid, params, ans=np.zeros(s.shape[0]))
+    return SYMMETRY[kid] * out
 
 
 def split_levels(flat, dim, P, depth):

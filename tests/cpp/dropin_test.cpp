@@ -157,6 +157,63 @@ int main()
         check("odd_mutual_vs_direct_3d", rel(a, b), 1e-13);
         check("odd_fmm_vs_exact_3d", rel(a, e), 1e-2);
     }
+    {  // differentiation with a user kernel and its gradient functor vs the built-in Gaussian
+        const size_t n = 20000;
+        auto t = pts<double, 3>(n), s = pts<double, 3>(n);
+        std::vector<double> w(n), cot(n);
+        for (auto &v : w) v = 2 * uni() - 1;
+        for (auto &v : cot) v = 2 * uni() - 1;
+        const double a = 1.0 / (2 * 0.1 * 0.1);
+        fmma::FMMA<double, 3> f;
+        f.set_fn([a] __host__ __device__(const std::array<double, 3> &d) {
+                     return std::exp(-a * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+                 },
+                 [a] __host__ __device__(const std::array<double, 3> &d) {
+                     const double e = -2 * a * std::exp(-a * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+                     return std::array<double, 3>{e * d[0], e * d[1], e * d[2]};
+                 });
+        f.set_solver_type("fmm");
+        f.set_poly_ord(3);
+        f.set_Depth(4);
+        f.set_trans_sym_flag(1);
+        fmma::FMMA<double, 3> g;
+        g.set_kernel(FMMA_KERNEL_GAUSS, {0.1});
+        g.set_solver_type("fmm");
+        g.set_poly_ord(3);
+        g.set_Depth(4);
+        g.set_trans_sym_flag(1);
+        std::vector<std::array<double, 3>> ga, gb;
+        std::vector<double> ta, tb;
+        f.gradient(t, w, s, ga);
+        g.gradient(t, w, s, gb);
+        f.solve_transpose(t, cot, s, ta);  // no symmetry declared: the transposed launcher table
+        g.solve_transpose(t, cot, s, tb);
+        std::vector<double> fa, fb;
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) {
+                fa.push_back(ga[i][d]);
+                fb.push_back(gb[i][d]);
+            }
+        check("user_grad_vs_builtin_gauss", rel(fa, fb), 1e-12);
+        check("user_transpose_vs_builtin_gauss", rel(ta, tb), 1e-12);
+    }
+    {  // transpose of a kernel that is neither symmetric nor translation invariant, exact solver, vs the host sum
+        const size_t nt = 300, ns = 200;
+        auto t = pts<double, 2>(nt), s = pts<double, 2>(ns);
+        std::vector<double> cot(nt), out;
+        for (auto &v : cot) v = 2 * uni() - 1;
+        auto fn = [] __host__ __device__(const std::array<double, 2> &x, const std::array<double, 2> &y) {
+            return (1.0 + x[0]) * std::exp(-(x[0] - y[0]) * (x[0] - y[0]) - 2.0 * (x[1] - y[1]) * (x[1] - y[1]));
+        };
+        fmma::FMMA<double, 2> f;
+        f.set_fn(fn);
+        f.set_solver_type("exact");
+        f.solve_transpose(t, cot, s, out);
+        std::vector<double> host(ns, 0.0);
+        for (size_t j = 0; j < ns; ++j)
+            for (size_t i = 0; i < nt; ++i) host[j] += cot[i] * fn(t[i], s[j]);
+        check("general_transpose_exact_vs_host", rel(out, host), 1e-13);
+    }
     {  // time log keys of the reference (src/fmm.cpp:57-141, src/fmma.cpp:189)
         fmma::FMMA<double, 2> f;
         auto x = pts<double, 2>(5000);

@@ -77,6 +77,10 @@ FMM_CASES = [
     dict(id="3d_ragged", dim=3, nt=1001, ns=777, shared=False, w="u01", p=3, depth=4, ts=1, k="inv_r_reg"),
     dict(id="2d_p7_d4", dim=2, nt=4000, ns=4000, shared=True, w="normal", p=7, depth=4, ts=1, k="log_r"),
     dict(id="3d_p5_d3", dim=3, nt=3000, ns=3000, shared=True, w="u01", p=5, depth=3, ts=1, k="inv_r_reg"),
+    # P = 343 and 512: the M2L contraction staged in several LDS passes
+    dict(id="3d_p6_d3", dim=3, nt=3000, ns=3000, shared=False, w="u01", p=6, depth=3, ts=1, k="inv_r_reg"),
+    dict(id="3d_p7_d3", dim=3, nt=2000, ns=2000, shared=True, w="normal", p=7, depth=3, ts=1, k="gauss",
+         params=(0.2,)),
 ]
 
 
@@ -239,3 +243,16 @@ def test_repeated_solves_are_deterministic(gpu):
     a, pl = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True)
     b, _ = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True, plan=pl)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_m2l_forms_at_512_coefficients(gpu, oracle, prec, tol):
+    """8 nodes per dimension in 3-D (P = 512): the contraction of the
+    non-pipelined M2L kernels takes several LDS passes."""
+    x = synth.points(75, 3000, 3)
+    w = synth.normal(76, 3000)
+    got, pl = gpu_solve(gpu, "fmm", x, w, x, 7, 3, 1, "inv_r_reg", shared=True, precision=prec)
+    ref = oracle.fmm(x, w, x, 7, 3, 1, KID["inv_r_reg"])
+    err = rel_l2(got, ref)
+    print("  P=512 %s vs oracle %.3e" % (prec, err))
+    assert err <= tol, err

@@ -1,0 +1,167 @@
+"""Pinning of the oracle's differentiation functions (CPU only).
+
+The reference has no gradients, so nothing of it can be compared directly.
+Each oracle function is pinned by what the mathematics fixes instead:
+
+* oracle_kernel_grad: central differences of oracle_kernel, per kernel id;
+* oracle_cheb_dS: closed forms (S_2(x, c) = 1/2 + x c has d/dx = c) and
+  central differences of oracle_cheb_S;
+* oracle_exact_grad: central differences of oracle_exact;
+* oracle_fmm_grad: central differences of oracle_fmm, which the compiled
+  reference pins (tests/test_oracle.py); targets near a box face are left out,
+  since the fmm approximation jumps across faces;
+* the transpose (fmm with targets and sources swapped): the adjoint identity
+  <g, A w> = <A^T g, w> with A the oracle fmm, for symmetric, antisymmetric and
+  non-translation-invariant kernels and both trans_sym_flag values.
+
+A plausible mistake (a dropped 2/len chain factor, a sign, a transposed
+offset, T_j' taken as U_j) fails one of these.
+"""
+import numpy as np
+import pytest
+
+from fmma_amd import synth
+from tests.conftest import rel_l2
+
+KID = {"inv_r": 0, "inv_r_reg": 1, "inv_diff": 2, "log_r": 3, "gauss": 4, "yukawa": 5, "sum_r2": 6, "sum2_r2": 7}
+
+
+@pytest.mark.parametrize("name,dim,params", [("inv_r_reg", 3, ()), ("inv_r", 2, ()), ("inv_diff", 1, ()),
+                                             ("log_r", 2, ()), ("gauss", 3, (0.3,)), ("yukawa", 3, (2.0,)),
+                                             ("sum_r2", 3, ()), ("sum2_r2", 2, ())])
+def test_kernel_grad_matches_central_differences(oracle, name, dim, params):
+    kid = KID[name]
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    for _ in range(20):
+        x, y = rng.random(dim), rng.random(dim)
+        if np.linalg.norm(x - y) < 0.1:
+            continue
+        g = oracle.kernel_grad(kid, params, x, y)
+        fd = np.zeros(dim)
+        for d in range(dim):
+            e = np.zeros(dim)
+            e[d] = h
+            fd[d] = (oracle.lib().oracle_kernel(kid, oracle._d(oracle._prm(params)), dim, oracle._d(x + e),
+                                                oracle._d(y)) -
+                     oracle.lib().oracle_kernel(kid, oracle._d(oracle._prm(params)), dim, oracle._d(x - e),
+                                                oracle._d(y))) / (2 * h)
+        assert np.allclose(g, fd, rtol=1e-6, atol=1e-7 * np.abs(fd).max()), (name, g, fd)
+
+
+def test_cheb_dS_closed_form_and_differences(oracle):
+    for c in (-0.7, 0.1, 0.9):
+        for x in (-0.5, 0.0, 0.6):
+            assert abs(oracle.cheb_dS(2, x, c) - c) < 1e-15          # S_2 = 1/2 + x c
+            # S_3(x, c) = 1/3 + (2/3)(x c + (2x^2 - 1)(2c^2 - 1)): d/dx = (2/3)(c + 4x(2c^2 - 1))
+            assert abs(oracle.cheb_dS(3, x, c) - 2 / 3 * (c + 4 * x * (2 * c * c - 1))) < 1e-14
+    h = 1e-6
+    for n in (4, 6, 9):
+        for x in np.linspace(-0.95, 0.95, 7):
+            for c in np.linspace(-0.9, 0.9, 5):
+                fd = (oracle.lib().oracle_cheb_S(n, x + h, c) - oracle.lib().oracle_cheb_S(n, x - h, c)) / (2 * h)
+                assert abs(oracle.cheb_dS(n, x, c) - fd) < 1e-7 * max(1.0, abs(fd))
+
+
+def _fd_all_targets(fun, t, h):
+    """Central differences of fun(targets) -> [N] in each dimension, moving
+    every target at once (each output depends on its own target only)."""
+    g = np.zeros_like(t)
+    for d in range(t.shape[1]):
+        e = np.zeros(t.shape[1])
+        e[d] = h
+        g[:, d] = (fun(t + e) - fun(t - e)) / (2 * h)
+    return g
+
+
+def _separated(dim, nt, ns, seed):
+    """Targets inside [0.1, 0.9]^dim; sources in [0, 1]^dim including the two
+    corners, so the bounding cube does not move with the targets."""
+    t = 0.1 + 0.8 * synth.points(seed, nt, dim)
+    s = synth.points(seed + 1, ns, dim)
+    s[0] = 0.0
+    s[1] = 1.0
+    return t, s
+
+
+@pytest.mark.parametrize("name,dim,params", [("inv_r_reg", 3, ()), ("log_r", 2, ()), ("gauss", 3, (0.1,)),
+                                             ("inv_diff", 1, ())])
+def test_exact_grad_matches_central_differences(oracle, name, dim, params):
+    t, s = _separated(dim, 300, 500 if dim > 1 else 40, 61)
+    w = synth.normal(63, s.shape[0])
+    g = oracle.exact_grad(t, w, s, KID[name], params)
+    fd = _fd_all_targets(lambda tt: oracle.exact(tt, w, s, KID[name], params), t, 1e-6)
+    # central differences lose accuracy next to a singular source: keep targets 1e-2 away
+    dmin = np.sqrt(((t[:, None, :] - s[None, :, :]) ** 2).sum(-1)).min(1)
+    keep = dmin > 1e-2
+    assert keep.sum() > 50
+    assert rel_l2(g[keep], fd[keep]) < 1e-7
+
+
+FMM_GRAD_CASES = [
+    dict(name="inv_r_reg", dim=3, p=3, depth=4, ts=1, params=()),
+    dict(name="inv_r_reg", dim=3, p=2, depth=3, ts=0, params=()),
+    dict(name="gauss", dim=3, p=4, depth=4, ts=1, params=(0.1,)),
+    dict(name="log_r", dim=2, p=5, depth=5, ts=1, params=()),
+    dict(name="yukawa", dim=2, p=4, depth=4, ts=1, params=(3.0,)),
+    dict(name="inv_diff", dim=1, p=7, depth=6, ts=1, params=()),
+    dict(name="sum2_r2", dim=2, p=4, depth=4, ts=0, params=()),
+]
+
+
+@pytest.mark.parametrize("case", FMM_GRAD_CASES, ids=lambda c: "%s_%dd_ts%d" % (c["name"], c["dim"], c["ts"]))
+def test_fmm_grad_matches_central_differences(oracle, case):
+    dim, depth = case["dim"], case["depth"]
+    t, s = _separated(dim, 400, 3000 if dim > 1 else 30, 71)
+    w = synth.normal(73, s.shape[0])
+    kid, p, ts, prm = KID[case["name"]], case["p"], case["ts"], case["params"]
+    g = oracle.fmm_grad(t, w, s, p, depth, ts, kid, prm)
+    h = 1e-6
+    fd = _fd_all_targets(lambda tt: oracle.fmm(tt, w, s, p, depth, ts, kid, prm), t, h)
+    # leave out targets within a few steps of a leaf-box face
+    _, rel = oracle.leaf_boxes(t, s, t, depth, rel=True)
+    o, Len = oracle.bbox(t, s)
+    margin = 8 * h / (Len / 2 ** (depth - 1)) * 2
+    keep = np.all(np.abs(rel) < 1 - margin, axis=1)
+    # and next to a singular source
+    keep &= np.sqrt(((t[:, None, :] - s[None, :, :]) ** 2).sum(-1)).min(1) > 1e-2
+    assert keep.sum() > 100
+    err = rel_l2(g[keep], fd[keep])
+    print(case["name"], dim, "fmm grad vs central differences %.2e" % err)
+    assert err < 1e-6, err
+    # and the fmm gradient approximates the exact one
+    ge = oracle.exact_grad(t, w, s, kid, prm)
+    assert rel_l2(g, ge) < 5e-2
+
+
+ADJ_CASES = [
+    dict(name="inv_r_reg", dim=3, p=3, depth=4, ts=1, params=(), shared=False),
+    dict(name="inv_r_reg", dim=3, p=2, depth=4, ts=0, params=(), shared=True),
+    dict(name="gauss", dim=3, p=3, depth=3, ts=1, params=(0.2,), shared=False),
+    dict(name="inv_diff", dim=1, p=7, depth=6, ts=1, params=(), shared=False),
+    dict(name="sum_r2", dim=2, p=4, depth=4, ts=1, params=(), shared=False),
+    dict(name="sum2_r2", dim=2, p=3, depth=4, ts=0, params=(), shared=False),
+    dict(name="log_r", dim=2, p=4, depth=2, ts=1, params=(), shared=False),
+]
+
+
+@pytest.mark.parametrize("case", ADJ_CASES, ids=lambda c: "%s_%dd_ts%d_d%d" % (c["name"], c["dim"], c["ts"],
+                                                                               c["depth"]))
+def test_fmm_transpose_is_the_adjoint(oracle, case):
+    dim, depth = case["dim"], case["depth"]
+    t, s = _separated(dim, 2500, 2000, 81)
+    if case["shared"]:
+        s = t
+    w = synth.normal(83, s.shape[0])
+    g = synth.normal(84, t.shape[0])
+    kid, p, ts, prm = KID[case["name"]], case["p"], case["ts"], case["params"]
+    Aw = oracle.fmm(t, w, s, p, depth, ts, kid, prm)
+    ATg = oracle.fmm_transpose(t, g, s, p, depth, ts, kid, prm)
+    lhs, rhs = float(g @ Aw), float(ATg @ w)
+    scale = float(np.abs(g) @ np.abs(Aw))
+    print(case["name"], "adjoint %.3e" % (abs(lhs - rhs) / scale))
+    assert abs(lhs - rhs) <= 1e-13 * scale
+    # and it approximates the exact transpose
+    ex = oracle.exact(s, g, t, kid, prm) * oracle.SYMMETRY[kid]
+    if np.all(np.isfinite(ex)):
+        assert rel_l2(ATg, ex) < 5e-2
