@@ -223,6 +223,45 @@ class Plan:
             raise ValueError("the output must be writeable")
         self.set_stream(0)
 
+    # ---- batches (new; the reference solves one problem per call)
+    def solve_batch(self, t_counts, targets, weights, s_counts, sources, ans):
+        """Independent problems in one pass: problem p owns t_counts[p]
+        consecutive rows of targets / ans and s_counts[p] consecutive rows of
+        sources / weights (GPU tensors, concatenated).  Each is solved as
+        solve() would solve it alone.  Passing the same tensor as targets and
+        sources (equal counts) makes every problem's targets its sources."""
+        tc = np.ascontiguousarray(t_counts, dtype=np.int64)
+        sc = np.ascontiguousarray(s_counts, dtype=np.int64)
+        if tc.ndim != 1 or sc.shape != tc.shape:
+            raise ValueError("t_counts and s_counts must be 1-D of equal length")
+        if targets.shape[0] != int(tc.sum()) or sources.shape[0] != int(sc.sum()):
+            raise ValueError("the counts must add up to the number of targets and sources")
+        self._check_vec(targets, weights, sources, ans, (targets.shape[0],), "ans")
+        if not _is_torch(targets):
+            raise ValueError("solve_batch takes GPU tensors")
+        self._torch_ready(targets, weights, sources, ans)
+        lp = ctypes.POINTER(ctypes.c_long)
+        _lib.check(self._L.fmma_plan_solve_batch(self._h, int(tc.size), tc.ctypes.data_as(lp), targets.data_ptr(),
+                                                 _dptr(weights), sc.ctypes.data_as(lp), sources.data_ptr(),
+                                                 ans.data_ptr()))
+        return ans
+
+    def set_batch_graph(self, on):
+        """Replay batched solves from one captured hipGraph (default on)."""
+        _lib.check(self._L.fmma_plan_set_batch_graph(self._h, 1 if on else 0))
+
+    def set_m2l_autotune(self, on):
+        """Time the M2L tile widths on the first batched solve of a layout (default on)."""
+        _lib.check(self._L.fmma_plan_set_m2l_autotune(self._h, 1 if on else 0))
+
+    def m2l_tiles(self):
+        """{parent boxes per level: M2L tile width} chosen by the autotuner."""
+        n = self._L.fmma_plan_m2l_tiles(self._h, None, None, 0)
+        par = (ctypes.c_long * max(n, 1))()
+        tb = (ctypes.c_int * max(n, 1))()
+        self._L.fmma_plan_m2l_tiles(self._h, par, tb, n)
+        return {int(par[i]): int(tb[i]) for i in range(n)}
+
     # ---- diagnostics
     def time_log(self):
         n = self._L.fmma_plan_time_log(self._h, None, None, 0)

@@ -21,7 +21,7 @@ ARCH = "gfx950"
 # (source, object name, extra flags).  builtins.hip is compiled once per
 # (TYPE, DIM) so its kernel instantiations build in parallel, and once more
 # as the dispatcher over them.
-SOURCES = [("tree.hip", "tree", []), ("boxops.hip", "boxops", []), ("m2l.hip", "m2l", []),
+SOURCES = [("tree.hip", "tree", []), ("boxops.hip", "boxops", []), ("m2l.hip", "m2l", []), ("batch.hip", "batch", []),
            ("plan.hip", "plan", []), ("builtins.hip", "builtins", [])] + [
     ("builtins.hip", "builtins_%s%d" % (t[0], d), ["-DFMMA_BI_TYPE=" + t, "-DFMMA_BI_DIM=%d" % d])
     for t in ("double", "float") for d in (1, 2, 3)]

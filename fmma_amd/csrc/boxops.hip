@@ -30,6 +30,20 @@ __device__ __forceinline__ void coeff_digits(int lane, int nn, int P, int (&dig)
     }
 }
 
+// The geometry and leaf level of a leaf box: the plan's, or in a batch of
+// independent problems those of the box's problem.
+template <typename T, class A>
+__device__ __forceinline__ int box_geom(const A &a, unsigned box, const Geom<T> *&g)
+{
+    if (a.box_prob) {
+        const unsigned p = a.box_prob[box];
+        g = a.geoms + p;
+        return a.levels[p];
+    }
+    g = a.geom;
+    return a.leaf_level;
+}
+
 // ---------------------------------------------------------------------- P2M
 
 template <typename T, int D, int KPL>
@@ -46,8 +60,9 @@ __global__ __launch_bounds__(64) void p2m_kernel(P2MArgs<T> a)
     T acc[KPL];
 #pragma unroll
     for (int r = 0; r < KPL; ++r) acc[r] = T(0);
-    const int NL = 1 << a.leaf_level;
-    const T len = a.geom->len_leaf;
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
     __syncthreads();
     for (unsigned base = s0; base < s1; base += 64) {
         const unsigned i = base + lane;
@@ -56,7 +71,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(P2MArgs<T> a)
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 T rel;
-                leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+                leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
                 cheb_weights(sc, nn, rel, d == 0 ? wt : T(1), sv + (lane * D + d) * nn, 1);
             }
         }
@@ -84,14 +99,14 @@ __global__ __launch_bounds__(64) void p2m_kernel(P2MArgs<T> a)
 // ---------------------------------------------------------------------- M2M
 
 template <typename T, int D, int KPL>
-__global__ __launch_bounds__(64) void m2m_kernel(const T *Wc, T *Wp, const T *tab, int nn, int P)
+__global__ __launch_bounds__(64) void m2m_kernel(const T *Wc, T *Wp, const T *tab, int nn, int P, const long *items)
 {
     __shared__ T st[2 * kMaxNodes * kMaxNodes];
     __shared__ T sw[(1 << D) * kMaxP];
     const unsigned parent = blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < 2 * nn * nn; i += 64) st[i] = tab[i];
-    const T *src = Wc + (size_t)parent * (1u << D) * P;
+    const T *src = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * (1u << D) * P);
     for (int i = lane; i < (1 << D) * P; i += 64) sw[i] = src[i];
     int dig[KPL][D];
     coeff_digits<D, KPL>(lane, nn, P, dig);
@@ -124,7 +139,7 @@ __global__ __launch_bounds__(64) void m2m_kernel(const T *Wc, T *Wp, const T *ta
             }
         }
         const int k = lane + 64 * r;
-        if (k < P) Wp[(size_t)parent * P + k] = acc;
+        if (k < P) Wp[(items ? (size_t)items[2 * parent + 1] : (size_t)parent * P) + k] = acc;
     }
 }
 
@@ -132,19 +147,19 @@ __global__ __launch_bounds__(64) void m2m_kernel(const T *Wc, T *Wp, const T *ta
 // L2L[s][ko][ki] = S_n((c_ko + 2s - 1)/2, c_ki) = tab[s][ki][ko] (S_n is symmetric).
 
 template <typename T, int D, int KPL>
-__global__ __launch_bounds__(64) void l2l_kernel(const T *Wp, T *Wc, const T *tab, int nn, int P)
+__global__ __launch_bounds__(64) void l2l_kernel(const T *Wp, T *Wc, const T *tab, int nn, int P, const long *items)
 {
     __shared__ T st[2 * kMaxNodes * kMaxNodes];
     __shared__ T sp[kMaxP];
     const unsigned parent = blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < 2 * nn * nn; i += 64) st[i] = tab[i];
-    for (int i = lane; i < P; i += 64) sp[i] = Wp[(size_t)parent * P + i];
+    for (int i = lane; i < P; i += 64) sp[i] = Wp[(items ? (size_t)items[2 * parent + 1] : (size_t)parent * P) + i];
     int dig[KPL][D];
     coeff_digits<D, KPL>(lane, nn, P, dig);
     __syncthreads();
     for (int sg = 0; sg < (1 << D); ++sg) {
-        T *dst = Wc + ((size_t)parent * (1u << D) + sg) * P;
+        T *dst = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * (1u << D) * P) + (size_t)sg * P;
 #pragma unroll
         for (int r = 0; r < KPL; ++r) {
             T acc = T(0);
@@ -190,8 +205,9 @@ __global__ __launch_bounds__(64) void l2p_kernel(L2PArgs<T> a)
     for (int i = lane; i < nn * nn; i += 64) sc[i] = a.coef[i];
     for (int i = lane; i < P; i += 64) sW[i] = a.Wl[(size_t)box * P + i];
     __syncthreads();
-    const int NL = 1 << a.leaf_level;
-    const T len = a.geom->len_leaf;
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
     T *S = sv + lane * D * nn;
     for (unsigned base = t0; base < t1; base += 64) {
         const unsigned i = base + lane;
@@ -199,7 +215,7 @@ __global__ __launch_bounds__(64) void l2p_kernel(L2PArgs<T> a)
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             T rel;
-            leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+            leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
             cheb_weights(sc, nn, rel, T(1), S + d * nn, 1);
         }
         T acc = T(0);
@@ -270,8 +286,9 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
     T acc[KPL];
 #pragma unroll
     for (int r = 0; r < KPL; ++r) acc[r] = T(0);
-    const int NL = 1 << a.leaf_level;
-    const T len = a.geom->len_leaf;
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
     __syncthreads();
     for (unsigned base = s0; base < s1; base += 64) {
         const unsigned i = base + lane;
@@ -280,7 +297,7 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 T rel;
-                leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+                leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
                 T out[NN];
                 cheb_weights_ct<T, NN>(sc, rel, d == 0 ? wt : T(1), out);
 #pragma unroll
@@ -328,7 +345,7 @@ __device__ __forceinline__ void apply_dim(const T *M, bool transpose, int d, con
 // prod_d S_n(c_ko, (c_ki + 2 sigma_d - 1)/2) is applied dimension by
 // dimension through LDS and summed over the children in registers.
 template <typename T, int D, int NN>
-__global__ __launch_bounds__(64) void m2m_ct(const T *Wc, T *Wp, const T *tab)
+__global__ __launch_bounds__(64) void m2m_ct(const T *Wc, T *Wp, const T *tab, const long *items)
 {
     constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
     __shared__ T st[2 * NN * NN];
@@ -337,7 +354,7 @@ __global__ __launch_bounds__(64) void m2m_ct(const T *Wc, T *Wp, const T *tab)
     const unsigned parent = blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < 2 * NN * NN; i += 64) st[i] = tab[i];
-    const T *src = Wc + (size_t)parent * NC * P;
+    const T *src = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * NC * P);
     for (int i = lane; i < NC * P; i += 64) sw[i] = src[i];
     T acc[KPL];
 #pragma unroll
@@ -374,13 +391,13 @@ __global__ __launch_bounds__(64) void m2m_ct(const T *Wc, T *Wp, const T *tab)
 #pragma unroll
     for (int r = 0; r < KPL; ++r) {
         const int k = lane + 64 * r;
-        if (k < P) Wp[(size_t)parent * P + k] = acc[r];
+        if (k < P) Wp[(items ? (size_t)items[2 * parent + 1] : (size_t)parent * P) + k] = acc[r];
     }
 }
 
 // L2L: one wave per parent; each child gets the transposed separable transfer.
 template <typename T, int D, int NN>
-__global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab)
+__global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab, const long *items)
 {
     constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
     __shared__ T st[2 * NN * NN];
@@ -388,7 +405,7 @@ __global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab)
     const unsigned parent = blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < 2 * NN * NN; i += 64) st[i] = tab[i];
-    for (int i = lane; i < P; i += 64) sp[i] = Wp[(size_t)parent * P + i];
+    for (int i = lane; i < P; i += 64) sp[i] = Wp[(items ? (size_t)items[2 * parent + 1] : (size_t)parent * P) + i];
     __syncthreads();
     for (int sg = 0; sg < NC; ++sg) {
         const T *cur = sp;
@@ -403,7 +420,7 @@ __global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab)
         }
         const T *M = st + ((sg >> (D - 1)) & 1) * NN * NN;
         const int stride = ipow_c(NN, D - 1);
-        T *dst = Wc + ((size_t)parent * NC + sg) * P;
+        T *dst = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * NC * P) + (size_t)sg * P;
 #pragma unroll
         for (int r = 0; r < KPL; ++r) {
             const int e = lane + 64 * r;
@@ -447,13 +464,14 @@ __global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
     if (i >= a.n) return;
     const unsigned box = a.keys[i];
     const T *W = staged ? sW + (size_t)(box - kb0) * P : a.Wl + (size_t)box * P;
-    const int NL = 1 << a.leaf_level;
-    const T len = a.geom->len_leaf;
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
     T S[D][NN];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         T rel;
-        leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, rel);
+        leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
         cheb_weights_ct<T, NN>(sc, rel, T(1), S[d]);
     }
     T acc = T(0);
@@ -497,15 +515,17 @@ static bool p2m_fast(const P2MArgs<T> &a, long nbox, hipStream_t st)
 }
 
 template <typename T, int D>
-static bool m2m_fast(const T *Wc, T *Wp, long nparent, const T *tab, int nn, hipStream_t st)
+static bool m2m_fast(const T *Wc, T *Wp, long nparent, const T *tab, int nn, hipStream_t st, const long *items)
 {
-    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((m2m_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wc, Wp, tab));
+    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((m2m_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wc, Wp, tab,
+                                          items));
 }
 
 template <typename T, int D>
-static bool l2l_fast(const T *Wp, T *Wc, long nparent, const T *tab, int nn, hipStream_t st)
+static bool l2l_fast(const T *Wp, T *Wc, long nparent, const T *tab, int nn, hipStream_t st, const long *items)
 {
-    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((l2l_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wp, Wc, tab));
+    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((l2l_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wp, Wc, tab,
+                                          items));
 }
 
 template <typename T, int D>
@@ -531,13 +551,14 @@ __global__ __launch_bounds__(256) void l2p_grad_kernel(L2PArgs<T> a)
     const int n = a.nn;
     const unsigned box = a.keys[i];
     const T *W = a.Wl + (size_t)box * a.P;
-    const int NL = 1 << a.leaf_level;
-    const T len = a.geom->len_leaf;
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
     T S[D][NMAX], dS[D][NMAX];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         T r;
-        leaf_coord_rel(a.pos[d * a.n + i], a.geom->origin[d], len, NL, r);
+        leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, r);
         for (int k = 0; k < n; ++k) {
             S[d][k] = a.coef[k * n];
             dS[d][k] = T(0);
@@ -634,23 +655,23 @@ int p2m(const P2MArgs<T> &a, long nbox, hipStream_t st)
 }
 
 template <typename T, int D>
-int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st)
+int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st, const long *items)
 {
     if (nparent <= 0) return FMMA_OK;
-    if (!m2m_fast<T, D>(Wc, Wp, nparent, tab, nn, st))
+    if (!m2m_fast<T, D>(Wc, Wp, nparent, tab, nn, st, items))
     FMMA_KPL_SWITCH(P, hipLaunchKernelGGL((m2m_kernel<T, D, KPL>), dim3((unsigned)nparent), dim3(64), 0, st, Wc,
-                                          Wp, tab, nn, P));
+                                          Wp, tab, nn, P, items));
     FMMA_LAUNCHED("m2m");
     return FMMA_OK;
 }
 
 template <typename T, int D>
-int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st)
+int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st, const long *items)
 {
     if (nparent <= 0) return FMMA_OK;
-    if (!l2l_fast<T, D>(Wp, Wc, nparent, tab, nn, st))
+    if (!l2l_fast<T, D>(Wp, Wc, nparent, tab, nn, st, items))
     FMMA_KPL_SWITCH(P, hipLaunchKernelGGL((l2l_kernel<T, D, KPL>), dim3((unsigned)nparent), dim3(64), 0, st, Wp,
-                                          Wc, tab, nn, P));
+                                          Wc, tab, nn, P, items));
     FMMA_LAUNCHED("l2l");
     return FMMA_OK;
 }
@@ -710,8 +731,8 @@ int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st)
     template int l2p_grad<T, D>(const L2PArgs<T> &, hipStream_t);                           \
     template int p2p_combine<T, D>(const P2PCombineArgs<T> &, hipStream_t);                 \
     template int p2m<T, D>(const P2MArgs<T> &, long, hipStream_t);                          \
-    template int m2m<T, D>(const T *, T *, long, const T *, int, int, hipStream_t);         \
-    template int l2l<T, D>(const T *, T *, long, const T *, int, int, hipStream_t);         \
+    template int m2m<T, D>(const T *, T *, long, const T *, int, int, hipStream_t, const long *); \
+    template int l2l<T, D>(const T *, T *, long, const T *, int, int, hipStream_t, const long *); \
     template int l2p<T, D>(const L2PArgs<T> &, long, hipStream_t);
 FMMA_BOX_INST(double, 1)
 FMMA_BOX_INST(double, 2)

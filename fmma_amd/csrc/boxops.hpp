@@ -16,6 +16,10 @@ struct P2MArgs {
     const T *coef;          // [nn][nn] Chebyshev weight coefficients
     int nn, P;
     T *Wm;                  // [nbox][P] leaf multipoles (written for every box)
+    // a batch of problems (else NULL): problem of each leaf box, per-problem geometry and leaf level
+    const unsigned *box_prob;
+    const Geom<T> *geoms;
+    const int *levels;
 };
 
 template <typename T>
@@ -30,6 +34,9 @@ struct L2PArgs {
     int nn, P;
     const T *Wl;            // [nbox][P] leaf locals
     T *out;                 // [n] sorted order
+    const unsigned *box_prob;  // batch (else NULL), as in P2MArgs
+    const Geom<T> *geoms;
+    const int *levels;
 };
 
 // Final sums of the mutual P2P form (fmma_p2p_args::slots).
@@ -47,12 +54,17 @@ template <typename T, int D>
 int p2m(const P2MArgs<T> &a, long nbox, hipStream_t st);
 
 // Wp[p] = M2M of children Wc[8p .. 8p+7] for p < nparent (overwrites Wp).
+// With items (a batch): parent q reads the 2^D child rows at Wc + items[2q]
+// and writes Wp + items[2q+1] (element offsets).
 template <typename T, int D>
-int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st);
+int m2m(const T *Wc, T *Wp, long nparent, const T *tab, int nn, int P, hipStream_t st,
+        const long *items = nullptr);
 
-// Wc[child] += L2L of Wp[parent] for every child of every parent.
+// Wc[child] += L2L of Wp[parent] for every child of every parent; items as in m2m
+// (items[2q] the children, items[2q+1] the parent).
 template <typename T, int D>
-int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st);
+int l2l(const T *Wp, T *Wc, long nparent, const T *tab, int nn, int P, hipStream_t st,
+        const long *items = nullptr);
 
 template <typename T, int D>
 int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);

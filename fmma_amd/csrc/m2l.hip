@@ -22,6 +22,7 @@
 //   16x16x32 bf16:      A[i][k]: lane = i + 16*(k/8), element k%8;
 //                       B[k][j]: lane = j + 16*(k/8), element k%8; C as above.
 #include "m2l.hpp"
+#include "../../include/fmma/detail/m2l_layout.hpp"
 
 namespace fmma {
 namespace lib {
@@ -81,7 +82,8 @@ void OffsetTables::build(int dim)
 // Tasks level by level, deepest first.  A level's tile width is the widest
 // of 256/128/64 boxes that still gives >= 512 workgroups (two per CU), when a
 // pipelined kernel exists for it; tasks of equal width form one launch.
-std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups)
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
+                                    const long *w_off, const long *c_off, const long *k_off)
 {
     std::vector<M2LTask> t;
     groups.clear();
@@ -102,7 +104,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         for (unsigned p0 = 0; p0 < npar; p0 += tb)
             for (int c = 0; c < (1 << D); ++c) {
                 const unsigned np = npar - p0 < (unsigned)tb ? npar - p0 : (unsigned)tb;
-                t.push_back(M2LTask{l, c, p0, (int)np});
+                t.push_back(M2LTask{l, c, p0, (int)np, w_off[l], c_off[l], k_off[l]});
                 ++groups.back().count;
             }
     }
@@ -131,9 +133,8 @@ __global__ __launch_bounds__(256) void frag4_kernel(const T *Kraw, CT *Kf, int P
     if (e >= total) return;
     const long per = (long)MP * KP;
     const long f = e / per;
-    const int r = (int)(e - f * per);
-    const int lane = r & 63, s = (r >> 6) % (KP / 4), rb = (r >> 6) / (KP / 4);
-    const int row = rb * 16 + (lane & 15), k = 4 * s + (lane >> 4);
+    int row, k;
+    detail::frag_rowcol(FMMA_FRAG_F32, KP, e - f * per, row, k);
     Kf[e] = (row < P && k < P) ? (CT)Kraw[(f * P + row) * P + k] : CT(0);
 }
 
@@ -145,9 +146,8 @@ __global__ __launch_bounds__(256) void frag_bf16_kernel(const T *Kraw, __bf16 *K
     if (e >= total) return;
     const long per = (long)MP * KP;
     const long f = e / per;
-    const int r = (int)(e - f * per);
-    const int j = r & 7, lane = (r >> 3) & 63, s = (r >> 9) % (KP / 32), rb = (r >> 9) / (KP / 32);
-    const int row = rb * 16 + (lane & 15), k = 32 * s + 8 * (lane >> 4) + j;
+    int row, k;
+    detail::frag_rowcol(FMMA_FRAG_BF16, KP, e - f * per, row, k);
     const float v = (row < P && k < P) ? (float)Kraw[(f * P + row) * P + k] : 0.0f;
     const __bf16 h = (__bf16)v;
     Kh[e] = h;
@@ -227,8 +227,8 @@ __global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
     bool tvalid;
     target_coords<D>(task, tid, tc, tvalid);
 
-    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
-    const CT *Kf = static_cast<const CT *>(a.Kf) + a.kf_off[task.level];
+    const TW *Wm = static_cast<const TW *>(a.Wm) + task.w_off;
+    const CT *Kf = static_cast<const CT *>(a.Kf) + task.k_off;
     const int nks = KP / 4, nrb = a.MP / 16;
 
     TW acc[4][4];
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
         }
     }
     if (!active) return;
-    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+    TW *Wl = static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
         const int col = 16 * cb + (lane & 15);
@@ -312,9 +312,9 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
     bool tvalid;
     target_coords<D>(task, tid, tc, tvalid);
 
-    const TW *Wm = static_cast<const TW *>(a.Wm) + a.lvl_off[task.level];
-    const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + a.kf_off[task.level]);
-    const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
+    const TW *Wm = static_cast<const TW *>(a.Wm) + task.w_off;
+    const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
+    const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
     const int nks = KP / 32, nrb = a.MP / 16;
 
     TW acc[4][4];
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
         }
     }
     if (!active) return;
-    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+    TW *Wl = static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
         const int col = 16 * cb + (lane & 15);
@@ -419,8 +419,8 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
         for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((task.cls >> (D - 1 - d)) & 1);
     }
-    const ST *Wg = static_cast<const ST *>(a.Wc) + a.lvlc_off[task.level];
-    [[maybe_unused]] const ST *Wg_lo = static_cast<const ST *>(a.Wc_lo) + a.lvlc_off[task.level];
+    const ST *Wg = static_cast<const ST *>(a.Wc) + task.c_off;
+    [[maybe_unused]] const ST *Wg_lo = static_cast<const ST *>(a.Wc_lo) + task.c_off;
     const int *cls_f = a.cls_f + task.cls * a.n_cls;
     const int *cls_o = a.cls_o + task.cls * a.n_cls * D;
 
@@ -451,16 +451,16 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         const size_t f = (size_t)cls_f[i];
         const size_t base = (f * nrb + rb) * NKS * 64;
         if constexpr (MODE == 2) {
-            const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + a.kf_off[task.level]);
+            const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
             const bf8_t *Kl =
-                reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + a.kf_off[task.level]);
+                reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
 #pragma unroll
             for (int s = 0; s < NKS; ++s) {
                 afr[s] = Kh[base + s * 64 + lane];
                 afr_lo[s] = Kl[base + s * 64 + lane];
             }
         } else {
-            const CT *Kf = static_cast<const CT *>(a.Kf) + a.kf_off[task.level];
+            const CT *Kf = static_cast<const CT *>(a.Kf) + task.k_off;
 #pragma unroll
             for (int s = 0; s < NKS; ++s) afr[s] = Kf[base + s * 64 + lane];
         }
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         __syncthreads();
     }
     if (!active) return;
-    TW *Wl = static_cast<TW *>(a.Wl) + a.lvl_off[task.level];
+    TW *Wl = static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
         const int cc = 16 * cb + (lane & 15);
@@ -566,6 +566,8 @@ static bool launch_pipe_kp(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 {
     if (tb == 64) return launch_pipe_tb<T, D, MODE, KP, 64>(a, grid, st);
     if constexpr (D == 3) {
+        if (tb == 16) return launch_pipe_tb<T, D, MODE, KP, 16>(a, grid, st);
+        if (tb == 32) return launch_pipe_tb<T, D, MODE, KP, 32>(a, grid, st);
         if (tb == 128) return launch_pipe_tb<T, D, MODE, KP, 128>(a, grid, st);
         if (tb == 256) return launch_pipe_tb<T, D, MODE, KP, 256>(a, grid, st);
     }
@@ -591,6 +593,7 @@ bool m2l_pipe_supported(int D, int KP, int mode, int tb)
     if (!kp_ok || (mode == 2 && KP % 32) || (KP / (256 / tb)) % vec) return false;
     if (tb == 64) return true;
     if (D != 3) return false;
+    if (tb == 16 || tb == 32) return true;
     return mode == 0 ? tb == 128 : (tb == 128 || tb == 256);
 }
 

@@ -24,10 +24,15 @@ struct OffsetTables {
     void build(int dim);
 };
 
+// A workgroup's tile: the sigma = cls children of parents p0 .. p0+np-1 of a
+// level, with the element offsets of that level's multipoles / locals
+// (w_off, rows of P), operand copy (c_off, rows of KP) and operator
+// fragments (k_off) -- per problem in a batch.
 struct M2LTask {
     int level, cls;
     unsigned p0;
     int np;
+    long w_off, c_off, k_off;
 };
 
 // A run of tasks with one tile width (one launch).
@@ -36,7 +41,10 @@ struct M2LGroup {
     bool pipelined;
 };
 
-std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups);
+// Tasks of levels leaf_level .. 2 of one problem; the offsets are filled
+// from per-level tables (element offsets) given by the caller.
+std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
+                                    const long *w_off, const long *c_off, const long *k_off);
 bool m2l_pipe_supported(int D, int KP, int mode, int tb);
 
 // Kernel arguments of the MFMA contraction.
@@ -46,15 +54,12 @@ struct M2LKArgs {
     const int *cls_o;
     int n_cls;
     int P, MP, KP;              // MP = 16*ceil(P/16) rows, KP = padded contraction length
-    long lvl_off[kMaxLevels];   // element offset of level l in the Wm / Wl arrays
-    long kf_off[kMaxLevels];    // element offset of level l in the fragment arrays
     const void *Wm;
     void *Wl;
     const void *Kf;             // operator fragments (f64 / f32), or bf16 hi part
     const void *Kf_lo;          // bf16 lo part
     const void *Wc;             // multipoles as MFMA operands, rows of KP: TYPE / float / bf16 hi
     const void *Wc_lo;          // bf16 lo part (mode 2)
-    long lvlc_off[kMaxLevels];  // element offset of level l in Wc / Wc_lo
 };
 
 // Shape of the operator fragments for a precision: rows MP, contraction KP.

@@ -6,13 +6,16 @@
 //   bounding cube -> leaf keys -> stable radix sort -> box ranges -> gather
 //   -> P2M -> M2M (levels L-1..2) -> M2L (levels 2..L, one MFMA launch)
 //   -> L2L (levels 2..L-1) -> L2P -> P2P (writes ans in caller order).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "boxops.hpp"
 #include "builtins.hpp"
 #include "m2l.hpp"
@@ -74,6 +77,40 @@ static double host_S(int n, double x, double y)
     return s;
 }
 
+// The layout of a batch of independent problems (fmma_plan_solve_batch),
+// kept while the problem sizes and the plan's settings stay the same.
+struct BatchPlan {
+    std::vector<long> tc, sc;  // the key: point counts, shared, settings generation, M2L mode
+    bool shared = false;
+    unsigned long gen = 0;
+    int mode = -1;
+
+    int nprob = 0, Lmax = 0, bits = 0;
+    long nt = 0, ns = 0, NB = 0, rows = 0;
+    std::vector<int> L;                  // leaf level per problem
+    std::vector<unsigned> bbase;         // [nprob+1] first global leaf box
+    std::vector<long> row_off;           // [nprob][kMaxLevels] first row (of P) of a level
+    std::vector<long> step_first, step_count;  // M2M steps, then L2L steps: ranges of items
+    int n_m2m_steps = 0;
+    std::vector<int> ent_level, ent_prob;      // M2L operator sets (problem, level)
+    std::vector<M2LGroup> groups;
+    bool tuned = false;                  // tile widths chosen (autotuner or rule)
+    DBuf d_toff, d_soff, d_levels, d_bbase, d_box_prob, d_geoms, d_items, d_tasks, d_ent_level, d_ent_prob;
+
+    hipGraph_t graph = nullptr;          // the captured solve and every address it was captured with
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> g_bufs;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~BatchPlan() { drop_graph(); }
+};
+
 // "P2P sum" is the combine pass of the mutual P2P form (zero for the direct form)
 static const char *kStageNames[] = {"tree", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "P2P sum"};
 constexpr int kStages = 8;
@@ -98,8 +135,14 @@ struct Plan {
     DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
     DBuf t_keys, t_vals, t_keys_alt, t_vals_alt, t_hist, t_pos, t_start;
     DBuf Wm, Wl, Kraw, Kf, Kf_lo, Wc, Wc_lo, l2p_out, exact_part, p2p_slots;
-    DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks;
+    DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks, d_op_levels, d_op_prob;
     DBuf h_t, h_s, h_w, h_ans;
+
+    // batches: the layout, graph replay, the M2L tile autotuner and its choices
+    std::unique_ptr<BatchPlan> batch;
+    bool batch_graph = true, m2l_autotune = true;
+    std::map<long, int> tuned_tb;        // (dim, P, mode, parents per level) -> tile width
+    hipStream_t cap_stream = nullptr;     // private stream for graph capture
 
     // cached set-up keyed by (poly_ord, leaf level, mode)
     int set_n = -1, set_L = -1, set_mode = -1;
@@ -129,6 +172,8 @@ struct Plan {
 
     ~Plan()
     {
+        batch.reset();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         if (events_ready)
             for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -168,10 +213,16 @@ static int upload(DBuf &b, const void *src, size_t bytes, hipStream_t st)
     return FMMA_OK;
 }
 
-// Constant tables that depend on (poly_ord, dim, dtype, leaf level, M2L mode).
-static int setup(Plan &pl, int L)
+static int frag_format(int mode)
 {
-    const int n = pl.poly_ord + 1, D = pl.dim, P = pl.P(), mode = pl.mode();
+    return mode == 0 ? FMMA_FRAG_F64 : mode == 1 ? FMMA_FRAG_F32 : FMMA_FRAG_BF16;
+}
+
+// Constant tables that depend on (poly_ord, dim, dtype): nodes, Chebyshev
+// weight coefficients, M2M/L2L transfer tables, M2L offset tables.
+static int setup_basis(Plan &pl)
+{
+    const int n = pl.poly_ord + 1, D = pl.dim;
     hipStream_t st = pl.stream;
     int rc;
     if (pl.set_n != n || pl.offs.D != D) {
@@ -206,6 +257,17 @@ static int setup(Plan &pl, int L)
         pl.set_n = n;
         pl.set_L = -1;
     }
+    return FMMA_OK;
+}
+
+// ... and those that depend on the leaf level and the M2L mode.
+static int setup(Plan &pl, int L)
+{
+    const int n = pl.poly_ord + 1, D = pl.dim, P = pl.P(), mode = pl.mode();
+    (void)n;
+    hipStream_t st = pl.stream;
+    int rc;
+    if ((rc = setup_basis(pl))) return rc;
     if (pl.set_L != L || pl.set_mode != mode) {
         long off = 0;
         for (int l = 0; l <= L; ++l) {
@@ -233,7 +295,11 @@ static int setup(Plan &pl, int L)
             const size_t opsz = mode == 0 ? pl.tsize() : mode == 1 ? 4 : 2;
             if ((rc = pl.Wc.ensure(offc * opsz))) return rc;
             if (mode == 2 && (rc = pl.Wc_lo.ensure(offc * 2))) return rc;
-            std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups);
+            std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off);
+            std::vector<int> op_levels, op_prob(L - 1, 0);
+            for (int l = 2; l <= L; ++l) op_levels.push_back(l);
+            if ((rc = upload(pl.d_op_levels, op_levels.data(), op_levels.size() * sizeof(int), st))) return rc;
+            if ((rc = upload(pl.d_op_prob, op_prob.data(), op_prob.size() * sizeof(int), st))) return rc;
             pl.n_tasks = (int)tasks.size();
             if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
             FMMA_HIP(hipStreamSynchronize(st));
@@ -319,16 +385,49 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     if (L >= 2) {
         const bool sym = pl.trans_sym == 1 || pl.ops.translation_invariant;
         if (sym) {
-            for (int l = 2; l <= L; ++l) {
-                fmma_m2l_op_args oa{D, pl.dtype, n, P, pl.d_nodes.as<double>(), l,
-                                    &pl.geom.as<Geom<T>>()->Len, pl.d_far.as<int>(), pl.offs.n_far,
-                                    pl.Kraw.p};
-                if ((rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st))) return rc == FMMA_ERR_HIP ? rc : rc;
-                char *kf = pl.Kf.as<char>(), *kl = pl.Kf_lo.as<char>();
-                const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
-                if ((rc = m2l_fragments<T>(pl.Kraw.as<T>(), pl.offs.n_far, P, mode, kf + pl.kf_off[l] * esz,
-                                           mode == 2 ? kl + pl.kf_off[l] * esz : nullptr, st)))
-                    return rc;
+            // the operators of levels 2..L straight into fragment order
+            fmma_m2l_op_args oa{};
+            oa.dim = D;
+            oa.dtype = pl.dtype;
+            oa.n_nodes = n;
+            oa.P = P;
+            oa.nodes = pl.d_nodes.as<double>();
+            oa.Len = &pl.geom.as<Geom<T>>()->Len;
+            oa.offsets = pl.d_far.as<int>();
+            oa.n_off = pl.offs.n_far;
+            oa.format = frag_format(mode);
+            m2l_shape(P, mode, oa.MP, oa.KP);
+            oa.n_batch = L - 1;
+            oa.len_stride = 0;
+            oa.batch_level = pl.d_op_levels.as<int>();
+            oa.batch_problem = pl.d_op_prob.as<int>();
+            oa.Kf = pl.Kf.p;
+            oa.Kf_lo = pl.Kf_lo.p;
+            rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st);
+            if (rc == FMMA_ERR_UNSUPPORTED) {
+                // a launcher table without the fragment form: one level at a time through Kraw
+                for (int l = 2; l <= L; ++l) {
+                    fmma_m2l_op_args ra{};
+                    ra.dim = D;
+                    ra.dtype = pl.dtype;
+                    ra.n_nodes = n;
+                    ra.P = P;
+                    ra.nodes = pl.d_nodes.as<double>();
+                    ra.level = l;
+                    ra.Len = &pl.geom.as<Geom<T>>()->Len;
+                    ra.offsets = pl.d_far.as<int>();
+                    ra.n_off = pl.offs.n_far;
+                    ra.K = pl.Kraw.p;
+                    ra.format = FMMA_FRAG_RAW;
+                    if ((rc = pl.ops.m2l_op(&ra, pl.ops.ctx, st))) return rc;
+                    char *kf = pl.Kf.as<char>(), *kl = pl.Kf_lo.as<char>();
+                    const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
+                    if ((rc = m2l_fragments<T>(pl.Kraw.as<T>(), pl.offs.n_far, P, mode, kf + pl.kf_off[l] * esz,
+                                               mode == 2 ? kl + pl.kf_off[l] * esz : nullptr, st)))
+                        return rc;
+                }
+            } else if (rc) {
+                return rc;
             }
             M2LKArgs ka{};
             ka.tasks = pl.d_tasks.as<M2LTask>();
@@ -337,17 +436,12 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
             ka.n_cls = pl.offs.n_cls;
             ka.P = P;
             m2l_shape(P, mode, ka.MP, ka.KP);
-            for (int l = 0; l < kMaxLevels; ++l) {
-                ka.lvl_off[l] = l <= L ? pl.lvl_off[l] : 0;
-                ka.kf_off[l] = pl.kf_off[l];
-            }
             ka.Wm = Wm;
             ka.Wl = Wl;
             ka.Kf = pl.Kf.p;
             ka.Kf_lo = pl.Kf_lo.p;
             ka.Wc = pl.Wc.p;
             ka.Wc_lo = pl.Wc_lo.p;
-            for (int l = 0; l < kMaxLevels; ++l) ka.lvlc_off[l] = l <= L ? pl.lvlc_off[l] : 0;
             {
                 const size_t opsz = mode == 0 ? sizeof(T) : mode == 1 ? 4 : 2;
                 char *wc = pl.Wc.as<char>() + pl.lvlc_off[2] * opsz;
@@ -591,6 +685,491 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
     return FMMA_OK;
 }
 
+// ------------------------------------------------------------------ batches
+//
+// fmma_plan_solve_batch: many independent problems of one configuration in
+// one pass of batched kernels.  Problem p is solved exactly as
+// fmma_plan_solve would solve it alone (its own bounding cube and Depth, the
+// auto rule applied to its own target count), so results agree to the last
+// bit with a loop of single solves.  Storage: the leaf multipoles/locals of
+// all problems sit in global leaf-box order (rows of P), then the coarser
+// levels of each problem; M2M/L2L run over item lists (child block, parent)
+// and M2L over tasks that carry their problem's row and operator offsets.
+
+static long tune_key(int D, int P, int mode, long npar)
+{
+    return (((long)D * 1024 + P) * 4 + mode) * (1L << 32) + npar;
+}
+
+// Tile width by rule: the narrowest pipelined width that covers a level's
+// parents, at most 64.
+static int rule_tb(int D, int KP, int mode, long npar)
+{
+    for (int tb : {16, 32, 64})
+        if (tb >= npar && m2l_pipe_supported(D, KP, mode, tb)) return tb;
+    return 64;
+}
+
+template <int D>
+static int batch_layout(Plan &pl, int nprob, const long *tc, const long *sc, bool shared)
+{
+    BatchPlan *bp = pl.batch.get();
+    const int mode = pl.mode(), P = pl.P();
+    if (bp && bp->gen == pl.gen && bp->mode == mode && bp->shared == shared && bp->nprob == nprob &&
+        std::equal(tc, tc + nprob, bp->tc.begin()) && std::equal(sc, sc + nprob, bp->sc.begin()))
+        return FMMA_OK;
+    pl.batch.reset(new BatchPlan());
+    bp = pl.batch.get();
+    bp->tc.assign(tc, tc + nprob);
+    bp->sc.assign(sc, sc + nprob);
+    bp->shared = shared;
+    bp->gen = pl.gen;
+    bp->mode = mode;
+    bp->nprob = nprob;
+    std::vector<long> toff(nprob + 1, 0), soff(nprob + 1, 0);
+    for (int p = 0; p < nprob; ++p) {
+        toff[p + 1] = toff[p] + tc[p];
+        soff[p + 1] = soff[p] + sc[p];
+    }
+    bp->nt = toff[nprob];
+    bp->ns = soff[nprob];
+    bp->L.resize(nprob);
+    bp->bbase.assign(nprob + 1, 0);
+    long nb = 0;
+    for (int p = 0; p < nprob; ++p) {
+        int depth = pl.depth > 0 ? pl.depth : (int)(std::log((double)tc[p]) / D) + 1;
+        if (depth <= 0) depth = 1;
+        const int L = depth - 1;
+        if (L < 2 || D * L > 30 || L >= kMaxLevels)
+            return fail(FMMA_ERR_UNSUPPORTED, "batch problem " + std::to_string(p) + ": Depth " +
+                                                  std::to_string(depth) + "; a batch needs 3 <= Depth and at most "
+                                                  "2^30 leaf boxes per problem");
+        bp->L[p] = L;
+        bp->bbase[p] = (unsigned)nb;
+        nb += 1L << (D * L);
+        bp->Lmax = std::max(bp->Lmax, L);
+        if (nb >= (1L << 31)) return fail(FMMA_ERR_UNSUPPORTED, "batch: more than 2^31 leaf boxes");
+    }
+    bp->bbase[nprob] = (unsigned)nb;
+    bp->NB = nb;
+    while ((1L << bp->bits) < nb) ++bp->bits;
+    bp->row_off.assign((size_t)nprob * kMaxLevels, -1);
+    long next = nb;
+    for (int p = 0; p < nprob; ++p) {
+        bp->row_off[(size_t)p * kMaxLevels + bp->L[p]] = bp->bbase[p];
+        for (int l = 2; l < bp->L[p]; ++l) {
+            bp->row_off[(size_t)p * kMaxLevels + l] = next;
+            next += 1L << (D * l);
+        }
+    }
+    bp->rows = next;
+    // M2M steps (parent level L_p - s), then L2L steps (parent level 2 + t)
+    std::vector<long> items;
+    auto add = [&](int p, int l) {
+        const long nq = 1L << (D * l);
+        const long crow = bp->row_off[(size_t)p * kMaxLevels + l + 1], prow = bp->row_off[(size_t)p * kMaxLevels + l];
+        for (long q = 0; q < nq; ++q) {
+            items.push_back((crow + (q << D)) * P);
+            items.push_back((prow + q) * P);
+        }
+    };
+    for (int st = 1; st <= bp->Lmax - 2; ++st) {
+        bp->step_first.push_back((long)items.size() / 2);
+        for (int p = 0; p < nprob; ++p)
+            if (bp->L[p] - st >= 2) add(p, bp->L[p] - st);
+        bp->step_count.push_back((long)items.size() / 2 - bp->step_first.back());
+    }
+    bp->n_m2m_steps = (int)bp->step_first.size();
+    for (int l = 2; l + 1 <= bp->Lmax; ++l) {
+        bp->step_first.push_back((long)items.size() / 2);
+        for (int p = 0; p < nprob; ++p)
+            if (l + 1 <= bp->L[p]) add(p, l);
+        bp->step_count.push_back((long)items.size() / 2 - bp->step_first.back());
+    }
+    for (int p = 0; p < nprob; ++p)
+        for (int l = 2; l <= bp->L[p]; ++l) {
+            bp->ent_level.push_back(l);
+            bp->ent_prob.push_back(p);
+        }
+    std::vector<unsigned> box_prob((size_t)nb);
+    for (int p = 0; p < nprob; ++p)
+        std::fill(box_prob.begin() + bp->bbase[p], box_prob.begin() + bp->bbase[p + 1], (unsigned)p);
+    hipStream_t st = pl.stream;
+    int rc;
+    if ((rc = upload(bp->d_toff, toff.data(), toff.size() * sizeof(long), st)) ||
+        (rc = upload(bp->d_soff, soff.data(), soff.size() * sizeof(long), st)) ||
+        (rc = upload(bp->d_levels, bp->L.data(), bp->L.size() * sizeof(int), st)) ||
+        (rc = upload(bp->d_bbase, bp->bbase.data(), bp->bbase.size() * sizeof(unsigned), st)) ||
+        (rc = upload(bp->d_box_prob, box_prob.data(), box_prob.size() * sizeof(unsigned), st)) ||
+        (rc = upload(bp->d_items, items.data(), items.size() * sizeof(long), st)) ||
+        (rc = upload(bp->d_ent_level, bp->ent_level.data(), bp->ent_level.size() * sizeof(int), st)) ||
+        (rc = upload(bp->d_ent_prob, bp->ent_prob.data(), bp->ent_prob.size() * sizeof(int), st)))
+        return rc;
+    FMMA_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+    return FMMA_OK;
+}
+
+// M2L tasks of the batch; tb_of maps parents-per-level to tile width.  With
+// only_npar >= 0, just the tasks of levels with that many parents.
+static std::vector<M2LTask> batch_tasks(const Plan &pl, const BatchPlan &bp, int D, std::vector<M2LGroup> &groups,
+                                        const std::map<long, int> &tb_of, long only_npar = -1)
+{
+    const int P = pl.P(), mode = pl.mode();
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
+    const long fe = (long)pl.offs.n_far * MP * KP;
+    std::map<int, std::vector<M2LTask>> by_tb;
+    for (size_t e = 0; e < bp.ent_level.size(); ++e) {
+        const int l = bp.ent_level[e], p = bp.ent_prob[e];
+        const long npar = 1L << (D * (l - 1));
+        if (only_npar >= 0 && npar != only_npar) continue;
+        const int tb = tb_of.at(npar);
+        const long row = bp.row_off[(size_t)p * kMaxLevels + l];
+        for (long p0 = 0; p0 < npar; p0 += tb)
+            for (int c = 0; c < (1 << D); ++c)
+                by_tb[tb].push_back(M2LTask{l, c, (unsigned)p0, (int)std::min<long>(tb, npar - p0), row * P, row * KP,
+                                            (long)e * fe});
+    }
+    groups.clear();
+    std::vector<M2LTask> all;
+    for (auto &kv : by_tb) {
+        groups.push_back(M2LGroup{kv.first, (int)all.size(), (int)kv.second.size(),
+                                  m2l_pipe_supported(D, KP, mode, kv.first)});
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+    }
+    return all;
+}
+
+template <typename T, int D>
+static int batch_buffers(Plan &pl, BatchPlan &bp)
+{
+    const int P = pl.P(), mode = pl.mode();
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
+    const size_t ts = sizeof(T);
+    int rc;
+    auto tree_bufs = [&](long n, DBuf &k, DBuf &v, DBuf &ka, DBuf &va, DBuf &h, DBuf &pos, DBuf *wb,
+                         DBuf &start) -> int {
+        int r;
+        if ((r = k.ensure(n * 4)) || (r = v.ensure(n * 4)) || (r = ka.ensure(n * 4)) || (r = va.ensure(n * 4)) ||
+            (r = h.ensure(radix_hist_words(n) * 4)) || (r = pos.ensure((size_t)n * D * ts)) ||
+            (r = start.ensure(((size_t)bp.NB + 1) * 4)))
+            return r;
+        if (wb && (r = wb->ensure((size_t)n * ts))) return r;
+        return FMMA_OK;
+    };
+    if ((rc = tree_bufs(bp.ns, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt, pl.s_hist, pl.s_pos, &pl.s_w,
+                        pl.s_start)))
+        return rc;
+    if (!bp.shared && (rc = tree_bufs(bp.nt, pl.t_keys, pl.t_vals, pl.t_keys_alt, pl.t_vals_alt, pl.t_hist, pl.t_pos,
+                                      nullptr, pl.t_start)))
+        return rc;
+    const size_t opsz = mode == 0 ? ts : mode == 1 ? 4 : 2;
+    const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
+    const size_t fe = (size_t)pl.offs.n_far * MP * KP;
+    constexpr int NU = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
+    if ((rc = bp.d_geoms.ensure((size_t)bp.nprob * sizeof(Geom<T>))) || (rc = pl.Wm.ensure(bp.rows * P * ts)) ||
+        (rc = pl.Wl.ensure(bp.rows * P * ts)) || (rc = pl.Wc.ensure(bp.rows * KP * opsz)) ||
+        (mode == 2 && (rc = pl.Wc_lo.ensure(bp.rows * KP * 2))) ||
+        (rc = pl.Kf.ensure(bp.ent_level.size() * fe * esz)) ||
+        (mode == 2 && (rc = pl.Kf_lo.ensure(bp.ent_level.size() * fe * esz))) ||
+        (rc = pl.l2p_out.ensure((size_t)bp.nt * ts)) || (rc = pl.p2p_slots.ensure((size_t)(NU + 1) * bp.nt * ts)))
+        return rc;
+    return FMMA_OK;
+}
+
+// Sort one point set of the batch into its global leaf boxes.
+template <typename T, int D>
+static int batch_tree(Plan &pl, BatchPlan &bp, const T *pts, const T *w, long n, const long *d_off, bool with_w,
+                      DBuf &keys, DBuf &vals, DBuf &keys_alt, DBuf &vals_alt, DBuf &hist, DBuf &pos, DBuf *wbuf,
+                      DBuf &start, hipStream_t st)
+{
+    int rc;
+    if ((rc = batch_keys<T, D>(pts, n, d_off, bp.nprob, bp.d_geoms.as<Geom<T>>(), bp.d_levels.as<int>(),
+                               bp.d_bbase.as<unsigned>(), keys.as<unsigned>(), st)))
+        return rc;
+    SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
+                   hist.as<unsigned>()};
+    if ((rc = radix_sort(sb, n, bp.bits, st))) return rc;
+    if ((rc = box_ranges(keys.as<unsigned>(), n, (unsigned)bp.NB, start.as<unsigned>(), st))) return rc;
+    return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, st);
+}
+
+// The batched solve as a sequence of kernels on st (capturable: no
+// allocation, copy from the host or synchronisation, except while tuning).
+template <typename T, int D>
+static int batch_kernels(Plan &pl, BatchPlan &bp, const T *tg, const T *w, const T *sr, T *ans, hipStream_t st,
+                         bool eager, bool tune)
+{
+    const int n = pl.poly_ord + 1, P = pl.P(), mode = pl.mode();
+    int rc;
+    auto mark = [&](int i) {
+        if (eager) pl.mark(i);
+    };
+    Geom<T> *geoms = bp.d_geoms.as<Geom<T>>();
+    mark(0);
+    if ((rc = batch_bbox<T, D>(tg, bp.d_toff.as<long>(), bp.shared ? nullptr : sr, bp.d_soff.as<long>(),
+                               bp.d_levels.as<int>(), bp.nprob, geoms, st)))
+        return rc;
+    if ((rc = batch_tree<T, D>(pl, bp, sr, w, bp.ns, bp.d_soff.as<long>(), true, pl.s_keys, pl.s_vals, pl.s_keys_alt,
+                               pl.s_vals_alt, pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start, st)))
+        return rc;
+    const T *t_pos = pl.s_pos.as<T>();
+    const unsigned *t_start = pl.s_start.as<unsigned>(), *t_perm = pl.s_vals.as<unsigned>(),
+                   *t_keys = pl.s_keys.as<unsigned>();
+    if (!bp.shared) {
+        if ((rc = batch_tree<T, D>(pl, bp, tg, nullptr, bp.nt, bp.d_toff.as<long>(), false, pl.t_keys, pl.t_vals,
+                                   pl.t_keys_alt, pl.t_vals_alt, pl.t_hist, pl.t_pos, nullptr, pl.t_start, st)))
+            return rc;
+        t_pos = pl.t_pos.as<T>();
+        t_start = pl.t_start.as<unsigned>();
+        t_perm = pl.t_vals.as<unsigned>();
+        t_keys = pl.t_keys.as<unsigned>();
+    }
+    mark(1);
+    T *Wm = pl.Wm.as<T>(), *Wl = pl.Wl.as<T>();
+    const unsigned *box_prob = bp.d_box_prob.as<unsigned>();
+    P2MArgs<T> pa{pl.s_start.as<unsigned>(), pl.s_pos.as<T>(), pl.s_w.as<T>(), bp.ns, nullptr, 0,
+                  pl.d_coef.as<T>(), n, P, Wm, box_prob, geoms, bp.d_levels.as<int>()};
+    if ((rc = p2m<T, D>(pa, bp.NB, st))) return rc;
+    mark(2);
+    const long *items = bp.d_items.as<long>();
+    for (int s = 0; s < bp.n_m2m_steps; ++s)
+        if ((rc = m2m<T, D>(Wm, Wm, bp.step_count[s], pl.d_tab.as<T>(), n, P, st, items + 2 * bp.step_first[s])))
+            return rc;
+    mark(3);
+    // M2L: every (problem, level) operator set straight into fragment order, then the contraction
+    fmma_m2l_op_args oa{};
+    oa.dim = D;
+    oa.dtype = pl.dtype;
+    oa.n_nodes = n;
+    oa.P = P;
+    oa.nodes = pl.d_nodes.as<double>();
+    oa.Len = &geoms->Len;
+    oa.len_stride = (int)(sizeof(Geom<T>) / sizeof(T));
+    oa.offsets = pl.d_far.as<int>();
+    oa.n_off = pl.offs.n_far;
+    oa.format = frag_format(mode);
+    m2l_shape(P, mode, oa.MP, oa.KP);
+    oa.n_batch = (int)bp.ent_level.size();
+    oa.batch_level = bp.d_ent_level.as<int>();
+    oa.batch_problem = bp.d_ent_prob.as<int>();
+    oa.Kf = pl.Kf.p;
+    oa.Kf_lo = pl.Kf_lo.p;
+    if ((rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st)))
+        return rc == FMMA_ERR_UNSUPPORTED ? fail(rc, "batch: the kernel's launcher table has no fragment form") : rc;
+    if ((rc = m2l_convert<T>(Wm, bp.rows, P, oa.KP, mode, pl.Wc.p, pl.Wc_lo.p, st))) return rc;
+    M2LKArgs ka{};
+    ka.cls_f = pl.d_cls_f.as<int>();
+    ka.cls_o = pl.d_cls_o.as<int>();
+    ka.n_cls = pl.offs.n_cls;
+    ka.P = P;
+    m2l_shape(P, mode, ka.MP, ka.KP);
+    ka.Wm = Wm;
+    ka.Wl = Wl;
+    ka.Kf = pl.Kf.p;
+    ka.Kf_lo = pl.Kf_lo.p;
+    ka.Wc = pl.Wc.p;
+    ka.Wc_lo = pl.Wc_lo.p;
+    if (tune) {
+        // the autotuner: for each number of parents per level, time every
+        // pipelined tile width on this batch's own tasks and keep the fastest
+        std::map<long, int> choice;
+        for (int l = 2; l <= bp.Lmax; ++l) {
+            const long npar = 1L << (D * (l - 1));
+            const long key = tune_key(D, P, mode, npar);
+            if (pl.tuned_tb.count(key)) {
+                choice[npar] = pl.tuned_tb[key];
+                continue;
+            }
+            float best = 1e30f;
+            int best_tb = rule_tb(D, ka.KP, mode, npar);
+            for (int tb : {16, 32, 64, 128, 256}) {
+                if (!m2l_pipe_supported(D, ka.KP, mode, tb) || tb > 4 * npar) continue;
+                std::map<long, int> one{{npar, tb}};
+                std::vector<M2LGroup> g;
+                std::vector<M2LTask> t = batch_tasks(pl, bp, D, g, one, npar);
+                if (t.empty()) break;
+                DBuf dt;
+                if ((rc = upload(dt, t.data(), t.size() * sizeof(M2LTask), st))) return rc;
+                M2LKArgs a2 = ka;
+                a2.tasks = dt.as<M2LTask>();
+                if ((rc = m2l_apply<T, D>(a2, g, mode, st))) return rc;  // warm-up
+                hipEvent_t e0, e1;
+                FMMA_HIP(hipEventCreate(&e0));
+                FMMA_HIP(hipEventCreate(&e1));
+                FMMA_HIP(hipEventRecord(e0, st));
+                for (int r = 0; r < 3 && !rc; ++r) rc = m2l_apply<T, D>(a2, g, mode, st);
+                FMMA_HIP(hipEventRecord(e1, st));
+                FMMA_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FMMA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_tb = tb;
+                }
+            }
+            pl.tuned_tb[key] = best_tb;
+            choice[npar] = best_tb;
+        }
+        std::vector<M2LTask> tasks = batch_tasks(pl, bp, D, bp.groups, choice);
+        if ((rc = upload(bp.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
+        FMMA_HIP(hipStreamSynchronize(st));
+        bp.tuned = true;
+    }
+    ka.tasks = bp.d_tasks.as<M2LTask>();
+    if ((rc = m2l_apply<T, D>(ka, bp.groups, mode, st))) return rc;
+    mark(4);
+    for (size_t s = bp.n_m2m_steps; s < bp.step_first.size(); ++s)
+        if ((rc = l2l<T, D>(Wl, Wl, bp.step_count[s], pl.d_tab.as<T>(), n, P, st, items + 2 * bp.step_first[s])))
+            return rc;
+    mark(5);
+    L2PArgs<T> la{t_start, t_keys, t_pos, bp.nt, nullptr, 0, pl.d_coef.as<T>(), n, P, Wl, pl.l2p_out.as<T>(),
+                  box_prob, geoms, bp.d_levels.as<int>()};
+    if ((rc = l2p<T, D>(la, bp.NB, st))) return rc;
+    mark(6);
+    const bool mutual = bp.shared && pl.p2p_mutual && (pl.ops.symmetry == 1 || pl.ops.symmetry == -1);
+    fmma_p2p_args qa{};
+    qa.dim = D;
+    qa.dtype = pl.dtype;
+    qa.n_target_boxes = bp.NB;
+    qa.leaf_level = 0;
+    qa.t_start = t_start;
+    qa.s_start = pl.s_start.as<unsigned>();
+    qa.t_pos = t_pos;
+    qa.s_pos = pl.s_pos.as<T>();
+    qa.s_w = pl.s_w.as<T>();
+    qa.n_t = bp.nt;
+    qa.n_s = bp.ns;
+    qa.base = pl.l2p_out.as<T>();
+    qa.t_perm = t_perm;
+    qa.ans = ans;
+    qa.accumulate = 0;
+    qa.slots = mutual ? pl.p2p_slots.p : nullptr;
+    qa.symmetry = mutual ? pl.ops.symmetry : 0;
+    qa.box_problem = box_prob;
+    qa.problem_box_base = bp.d_bbase.as<unsigned>();
+    qa.problem_leaf_level = bp.d_levels.as<int>();
+    if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+    mark(7);
+    if (mutual) {
+        P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), bp.nt, pl.l2p_out.as<T>(), t_perm, ans, 0};
+        if ((rc = p2p_combine<T, D>(ca, st))) return rc;
+    }
+    mark(8);
+    return FMMA_OK;
+}
+
+static std::vector<const void *> batch_buffer_ptrs(const Plan &pl, const void *tg, const void *w, const void *sr,
+                                                   const void *ans)
+{
+    const BatchPlan &b = *pl.batch;
+    return {tg, w, sr, ans, pl.s_keys.p, pl.s_vals.p, pl.s_keys_alt.p, pl.s_vals_alt.p, pl.s_hist.p, pl.s_pos.p,
+            pl.s_w.p, pl.s_start.p, pl.t_keys.p, pl.t_vals.p, pl.t_keys_alt.p, pl.t_vals_alt.p, pl.t_hist.p,
+            pl.t_pos.p, pl.t_start.p, pl.Wm.p, pl.Wl.p, pl.Wc.p, pl.Wc_lo.p, pl.Kf.p, pl.Kf_lo.p, pl.l2p_out.p,
+            pl.p2p_slots.p, pl.d_nodes.p, pl.d_coef.p, pl.d_tab.p, pl.d_far.p, pl.d_cls_f.p, pl.d_cls_o.p,
+            b.d_toff.p, b.d_soff.p, b.d_levels.p, b.d_bbase.p, b.d_box_prob.p, b.d_geoms.p, b.d_items.p,
+            b.d_tasks.p, b.d_ent_level.p, b.d_ent_prob.p, pl.ops.ctx};
+}
+
+// The batch entry: layout, buffers, then an eager pass (which also runs the
+// autotuner the first time) or a replay of the captured graph.
+template <typename T, int D>
+static int batch_typed(Plan &pl, int nprob, const long *tc, const void *tg_, const void *w_, const long *sc,
+                       const void *sr_, void *ans_)
+{
+    const T *tg = (const T *)tg_, *w = (const T *)w_, *sr = (const T *)sr_;
+    T *ans = (T *)ans_;
+    const bool shared = tg_ == sr_ && std::equal(tc, tc + nprob, sc);
+    int rc;
+    if ((rc = setup_basis(pl))) return rc;
+    if ((rc = batch_layout<D>(pl, nprob, tc, sc, shared))) return rc;
+    BatchPlan &bp = *pl.batch;
+    if ((rc = batch_buffers<T, D>(pl, bp))) return rc;
+    if (!bp.tuned) {
+        if (!pl.m2l_autotune) {
+            std::map<long, int> choice;
+            int MP, KP;
+            m2l_shape(pl.P(), pl.mode(), MP, KP);
+            for (int l = 2; l <= bp.Lmax; ++l) {
+                const long npar = 1L << (D * (l - 1));
+                choice[npar] = rule_tb(D, KP, pl.mode(), npar);
+            }
+            std::vector<M2LTask> tasks = batch_tasks(pl, bp, D, bp.groups, choice);
+            if ((rc = upload(bp.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), pl.stream))) return rc;
+            FMMA_HIP(hipStreamSynchronize(pl.stream));
+            bp.tuned = true;
+        } else {
+            return batch_kernels<T, D>(pl, bp, tg, w, sr, ans, pl.stream, true, true);
+        }
+    }
+    if (!pl.batch_graph) return batch_kernels<T, D>(pl, bp, tg, w, sr, ans, pl.stream, true, false);
+    // the graph holds every buffer address: capture again when any moved
+    const std::vector<const void *> bufs = batch_buffer_ptrs(pl, tg_, w_, sr_, ans_);
+    if (!bp.exec || bufs != bp.g_bufs) {
+        bp.drop_graph();
+        if (!pl.cap_stream) FMMA_HIP(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+        FMMA_HIP(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_kernels<T, D>(pl, bp, tg, w, sr, ans, pl.cap_stream, false, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(pl.cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "capture of the batched solve");
+        bp.graph = g;
+        FMMA_HIP(hipGraphInstantiateWithFlags(&bp.exec, bp.graph, 0));
+        bp.g_bufs = bufs;
+    }
+    pl.mark(0);
+    FMMA_HIP(hipGraphLaunch(bp.exec, pl.stream));
+    pl.mark(kStages);
+    return FMMA_OK;
+}
+
+static int run_batch(Plan &pl, int nprob, const long *tc, const void *tg, const void *w, const long *sc,
+                     const void *sr, void *ans)
+{
+    if (pl.solver != "fmm") return fail(FMMA_ERR_UNSUPPORTED, "batches run the fmm solver only");
+    if (nprob <= 0) return FMMA_OK;
+    if (!tc || !sc || !tg || !sr || !ans) return fail(FMMA_ERR_INVALID, "null pointer");
+    for (int p = 0; p < nprob; ++p)
+        if (tc[p] <= 0 || sc[p] <= 0) return fail(FMMA_ERR_INVALID, "every problem of a batch needs points");
+    if (!(pl.trans_sym == 1 || pl.ops.translation_invariant))
+        return fail(FMMA_ERR_UNSUPPORTED, "batches need per-offset M2L operators (trans_sym_flag 1 or a "
+                                          "translation-invariant kernel)");
+    if (!pl.ops.p2p || pl.ops.dim != pl.dim || pl.ops.dtype != pl.dtype)
+        return fail(FMMA_ERR_INVALID, "kernel launchers do not match the plan's DIM/TYPE");
+    if (pl.timing && !pl.events_ready) {
+        for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
+        pl.events_ready = true;
+    }
+    pl.view.valid = false;  // the batch reuses the tree buffers of single solves
+    pl.last_L = -1;
+    int rc;
+    if (pl.dtype == FMMA_F64) {
+        if (pl.dim == 1) rc = batch_typed<double, 1>(pl, nprob, tc, tg, w, sc, sr, ans);
+        else if (pl.dim == 2) rc = batch_typed<double, 2>(pl, nprob, tc, tg, w, sc, sr, ans);
+        else rc = batch_typed<double, 3>(pl, nprob, tc, tg, w, sc, sr, ans);
+    } else {
+        if (pl.dim == 1) rc = batch_typed<float, 1>(pl, nprob, tc, tg, w, sc, sr, ans);
+        else if (pl.dim == 2) rc = batch_typed<float, 2>(pl, nprob, tc, tg, w, sc, sr, ans);
+        else rc = batch_typed<float, 3>(pl, nprob, tc, tg, w, sc, sr, ans);
+    }
+    if (rc) return rc;
+    FMMA_HIP(hipStreamSynchronize(pl.stream));
+    if (pl.timing && pl.events_ready) {
+        float total = 0.f;
+        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
+        pl.time_log["total time(fmm batch)"] = total;
+    }
+    return FMMA_OK;
+}
+
 static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans)
 {
     return run_op(pl, Op::Solve, nt, tg, w, ns, sr, ans, false);
@@ -755,6 +1334,43 @@ int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable)
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.p2p_mutual = enable != 0;
     return FMMA_OK;
+}
+
+int fmma_plan_set_batch_graph(fmma_plan plan, int enable)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.batch_graph = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.m2l_autotune = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_m2l_tiles(fmma_plan plan, long *parents, int *tile, int max)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    const Plan &pl = plan->pl;
+    int i = 0;
+    for (const auto &kv : pl.tuned_tb) {
+        if (i < max && parents && tile) {
+            parents[i] = kv.first & 0xffffffffL;
+            tile[i] = kv.second;
+        }
+        ++i;
+    }
+    return i;
+}
+
+int fmma_plan_solve_batch(fmma_plan plan, int n_problems, const long *t_counts, const void *targets,
+                          const void *weights, const long *s_counts, const void *sources, void *ans)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    FMMA_GUARD(return fmma::lib::run_batch(plan->pl, n_problems, t_counts, targets, weights, s_counts, sources,
+                                           ans));
 }
 
 int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,

@@ -51,6 +51,17 @@ def gaussian_mixture(seed, n, dim, components=16, sigma=0.05):
     return centres[comp] + sigma * z
 
 
+def batch_counts(seed, nprob, lo=1024, hi=65536):
+    """nprob point counts, log-uniform in [lo, hi]."""
+    u = uniform(seed, nprob)
+    return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64).clip(lo, hi)
+
+
+def batch_points(seed, counts, dim):
+    """Concatenated points, uniform in each problem's own unit cube."""
+    return points(seed, int(np.sum(counts)), dim)
+
+
 # The five benchmark workloads, as (name, description) -- parameters follow
 # the task's configuration list; bench.py and the tests build them from these.
 def config(name, n=None):
@@ -78,4 +89,10 @@ def config(name, n=None):
         x = points(1, n, 3)
         return dict(dim=3, poly_ord=4, depth=-1, kernel="yukawa", params=(2.0,), trans_sym=1,
                     targets=x, sources=x, weights=normal(2, n))
+    if name == "c5_batch":
+        nprob = n or 512
+        counts = batch_counts(7, nprob)
+        x = batch_points(8, counts, 3)
+        return dict(dim=3, poly_ord=4, depth=-1, kernel="yukawa", params=(2.0,), trans_sym=1, counts=counts,
+                    targets=x, sources=x, weights=normal(9, x.shape[0]))
     raise KeyError(name)

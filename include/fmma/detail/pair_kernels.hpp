@@ -22,6 +22,7 @@
 #include <utility>
 
 #include "../../fmma_hip.h"
+#include "m2l_layout.hpp"
 #include "morton.hpp"
 
 namespace fmma {
@@ -92,6 +93,20 @@ struct Transposed {
         return k.template operator()<T, D>(y, x);
     }
 };
+
+// The leaf grid of a box: the plan's, or in a batch of independent problems
+// that of the box's problem, whose boxes start at gbase (Morton order).
+template <class A>
+__device__ __forceinline__ int leaf_grid(const A &a, unsigned box, unsigned &gbase)
+{
+    if (a.box_problem) {
+        const unsigned p = a.box_problem[box];
+        gbase = a.problem_box_base[p];
+        return a.problem_leaf_level[p];
+    }
+    gbase = 0;
+    return a.leaf_level;
+}
 
 // ----------------------------------------------------------------- P2P
 //
@@ -164,8 +179,9 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
     unsigned cnt_nb = 0;
     if (lane < (unsigned)NNB) {
         int bc[D], m[D];
-        morton_decode<D>(box, bc);
-        const int NL = 1 << a.leaf_level;
+        unsigned gbase;
+        const int NL = 1 << leaf_grid(a, box, gbase);
+        morton_decode<D>(box - gbase, bc);
         bool ok = true;
         constexpr int centre = (NNB - 1) / 2;
         int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
@@ -177,7 +193,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
         }
         unsigned first = 0;
         if (ok) {
-            const unsigned mm = morton_encode<D>(m);
+            const unsigned mm = gbase + morton_encode<D>(m);
             first = a.s_start[mm];
             cnt_nb = a.s_start[mm + 1] - first;
         }
@@ -409,8 +425,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
     if (lane <= (unsigned)(2 * NU)) {
         const bool upper = lane <= (unsigned)NU;
         int bc[D], m[D];
-        morton_decode<D>(box, bc);
-        const int NL = 1 << a.leaf_level;
+        unsigned gbase;
+        const int NL = 1 << leaf_grid(a, box, gbase);
+        morton_decode<D>(box - gbase, bc);
         bool ok = true;
         int k = lane == 0 ? NU : upper ? NU + (int)lane : (int)lane;
         const int sgn = upper ? 1 : -1;
@@ -422,7 +439,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
         }
         unsigned first = 0, cnt = 0;
         if (ok) {
-            const unsigned mm = morton_encode<D>(m);
+            const unsigned mm = gbase + morton_encode<D>(m);
             first = a.s_start[mm];
             cnt = a.s_start[mm + 1] - first;
         }
@@ -607,8 +624,9 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
     unsigned cnt_nb = 0;
     if (lane < (unsigned)NNB) {
         int bc[D], m[D];
-        morton_decode<D>(box, bc);
-        const int NL = 1 << a.leaf_level;
+        unsigned gbase;
+        const int NL = 1 << leaf_grid(a, box, gbase);
+        morton_decode<D>(box - gbase, bc);
         bool ok = true;
         constexpr int centre = (NNB - 1) / 2;
         int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
@@ -620,7 +638,7 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
         }
         unsigned first = 0;
         if (ok) {
-            const unsigned mm = morton_encode<D>(m);
+            const unsigned mm = gbase + morton_encode<D>(m);
             first = a.s_start[mm];
             cnt_nb = a.s_start[mm + 1] - first;
         }
@@ -833,14 +851,10 @@ int launch_exact(const fmma_exact_args *a, const K &k, hipStream_t s)
 // TYPE (src/fmm_core.cpp:405-414).  Multi-index digits: dimension 0 most
 // significant (src/fmm_core.cpp:246-253).
 template <typename T, int D, class K>
-__global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
+__device__ __forceinline__ T m2l_op_value(const fmma_m2l_op_args &a, T Len, int level, int oi, int kl, int km,
+                                          const K &kern)
 {
-    const int oi = blockIdx.y;
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    const long PP = (long)a.P * a.P;
-    if (e >= PP) return;
-    const int kl = (int)(e / a.P), km = (int)(e % a.P);
-    const T len = *static_cast<const T *>(a.Len) / (T)(1u << a.level);
+    const T len = Len / (T)(1u << level);
     T X[D], Y[D];
     int dl = kl, dm = km;
 #pragma unroll
@@ -851,15 +865,65 @@ __global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
         X[d] = len * (cl / 2);
         Y[d] = len * ((T)a.offsets[oi * D + d] + cm / 2);
     }
-    static_cast<T *>(a.K)[oi * PP + e] = kern.template operator()<T, D>(X, Y);
+    return kern.template operator()<T, D>(X, Y);
+}
+
+template <typename T, int D, class K>
+__global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
+{
+    const int oi = blockIdx.y;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long PP = (long)a.P * a.P;
+    if (e >= PP) return;
+    const int kl = (int)(e / a.P), km = (int)(e % a.P);
+    static_cast<T *>(a.K)[oi * PP + e] = m2l_op_value<T, D>(a, *static_cast<const T *>(a.Len), a.level, oi, kl, km, kern);
+}
+
+// Operator sets of several (problem, level) pairs straight into fragment
+// order, with the same values and roundings as m2l_op_kernel followed by the
+// library's fragment conversion.
+template <typename T, int D, class K>
+__global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K kern)
+{
+    const int oi = blockIdx.y, b = blockIdx.z;
+    const long per = (long)a.MP * a.KP;
+    const long r = (long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= per) return;
+    int row, col;
+    frag_rowcol(a.format, a.KP, r, row, col);
+    T v = T(0);
+    if (row < a.P && col < a.P) {
+        const T Len = static_cast<const T *>(a.Len)[(long)a.batch_problem[b] * a.len_stride];
+        v = m2l_op_value<T, D>(a, Len, a.batch_level[b], oi, row, col, kern);
+    }
+    const long e = ((long)b * a.n_off + oi) * per + r;
+    if (a.format == FMMA_FRAG_F64) {
+        static_cast<double *>(a.Kf)[e] = (double)v;
+    } else if (a.format == FMMA_FRAG_F32) {
+        static_cast<float *>(a.Kf)[e] = (float)v;
+    } else {
+        const float fv = (float)v;
+        const __bf16 h = (__bf16)fv;
+        static_cast<__bf16 *>(a.Kf)[e] = h;
+        static_cast<__bf16 *>(a.Kf_lo)[e] = (__bf16)(fv - (float)h);
+    }
 }
 
 template <typename T, int D, class K>
 int launch_m2l_op(const fmma_m2l_op_args *a, const K &k, hipStream_t s)
 {
-    const long PP = (long)a->P * a->P;
-    hipLaunchKernelGGL((m2l_op_kernel<T, D, K>), dim3((unsigned)((PP + 255) / 256), (unsigned)a->n_off), dim3(256),
-                       0, s, *a, k);
+    if (a->format == FMMA_FRAG_RAW) {
+        const long PP = (long)a->P * a->P;
+        hipLaunchKernelGGL((m2l_op_kernel<T, D, K>), dim3((unsigned)((PP + 255) / 256), (unsigned)a->n_off),
+                           dim3(256), 0, s, *a, k);
+    } else {
+        if (a->n_batch <= 0) return FMMA_OK;
+        if (a->format == FMMA_FRAG_F64 && sizeof(T) != 8) return FMMA_ERR_INVALID;
+        const long per = (long)a->MP * a->KP;
+        hipLaunchKernelGGL((m2l_op_frag_kernel<T, D, K>),
+                           dim3((unsigned)((per + 255) / 256), (unsigned)a->n_off, (unsigned)a->n_batch), dim3(256), 0,
+                           s, *a, k);
+    }
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }
 

@@ -90,6 +90,13 @@ typedef struct {
      * leaves ans alone; the library's combine pass adds them in a fixed order. */
     void *slots;               /* [n_upper+1][n_t] TYPE, or NULL for the direct form */
     int symmetry;              /* +1 or -1 in the mutual form */
+    /* A batch of independent problems (else NULL): leaf box b belongs to
+     * problem box_problem[b], whose boxes problem_box_base[p] ..
+     * problem_box_base[p+1]-1 are its Morton-ordered grid of side
+     * 1 << problem_leaf_level[p]; leaf_level is then unused. */
+    const unsigned *box_problem;
+    const unsigned *problem_box_base;
+    const int *problem_leaf_level;
 } fmma_p2p_args;
 
 typedef struct {
@@ -120,6 +127,9 @@ typedef struct {
     const void *base;          /* [dim][n_t] far-field gradient, sorted order, or NULL */
     const unsigned *t_perm;    /* [n_t] */
     void *grad;                /* [n_t][dim] caller's output, original order, overwritten */
+    const unsigned *box_problem;  /* a batch, as in fmma_p2p_args (else NULL) */
+    const unsigned *problem_box_base;
+    const int *problem_leaf_level;
 } fmma_p2p_grad_args;
 
 /* All-pairs gradient: grad[i][d] = sum_j w_j d/dx_d f(x_i, y_j). */
@@ -132,18 +142,29 @@ typedef struct {
     void *grad;                /* [n_t][dim] */
 } fmma_exact_grad_args;
 
+/* Element order of M2L operator fragments (include/fmma/detail/m2l_layout.hpp). */
+enum { FMMA_FRAG_RAW = 0, FMMA_FRAG_F64 = 1, FMMA_FRAG_F32 = 2, FMMA_FRAG_BF16 = 3 };
+
 /* The bounding cube lives on the device (it is reduced there); the box side
- * at a level is len = Len / (TYPE)(1 << level), as in src/fmm_core.cpp:376. */
+ * at a level is len = Len / (TYPE)(1 << level), as in src/fmm_core.cpp:376.
+ * format FMMA_FRAG_RAW: one level into K.  Otherwise n_batch operator sets,
+ * set b at level batch_level[b] with cube side Len[batch_problem[b] *
+ * len_stride], written straight into fragment order (MP rows, KP columns per
+ * offset) at Kf (+ Kf_lo for the BF16 lo part), set b at b * n_off * MP * KP. */
 typedef struct {
     int dim, dtype;
     int n_nodes;               /* nodes per dimension n = poly_ord + 1 */
     int P;                     /* n^dim */
     const double *nodes;       /* [n] Chebyshev nodes (device) */
-    int level;                 /* boxes per side 1 << level */
+    int level;                 /* boxes per side 1 << level (FMMA_FRAG_RAW) */
     const void *Len;           /* TYPE, device: side of the bounding cube */
     const int *offsets;        /* [n_off][dim] separation vectors o (device) */
     int n_off;
     void *K;                   /* [n_off][P][P] K[o][kl][km] = f(len*c_kl/2, len*(o + c_km/2)), TYPE */
+    int format, MP, KP, n_batch, len_stride;
+    const int *batch_level;    /* [n_batch] device */
+    const int *batch_problem;  /* [n_batch] device */
+    void *Kf, *Kf_lo;
 } fmma_m2l_op_args;
 
 typedef struct {
@@ -278,6 +299,29 @@ int fmma_plan_solve_transpose(fmma_plan plan, long n_t, const void *targets, con
                               long n_s, const void *sources, void *out);
 int fmma_plan_solve_transpose_host(fmma_plan plan, long n_t, const void *targets, const void *cot,
                                    long n_s, const void *sources, void *out);
+
+/* New (batching): nprob independent problems with this plan's settings in
+ * one pass of batched kernels.  Problem p has t_counts[p] targets and
+ * s_counts[p] sources, stored one after another in the concatenated device
+ * arrays targets [sum t_counts][dim], weights [sum s_counts] (NULL for
+ * all-ones), sources [sum s_counts][dim]; ans [sum t_counts] is overwritten.
+ * Each problem is solved as fmma_plan_solve would solve it alone: its own
+ * bounding cube, and its own Depth (the plan's when set, else the auto rule
+ * on t_counts[p]; each must be >= 3).  targets == sources with equal counts
+ * means every problem's targets are its sources.  Solver "fmm" with
+ * per-offset M2L operators (trans_sym_flag 1 or a translation-invariant
+ * kernel).  The first call for a layout also runs the M2L tile autotuner
+ * (fmma_plan_set_m2l_autotune); later calls replay one captured hipGraph
+ * (fmma_plan_set_batch_graph) while the layout, the settings and every
+ * address stay the same.  No reference counterpart: the reference solves one
+ * problem per call (src/fmm.cpp:15-144). */
+int fmma_plan_solve_batch(fmma_plan plan, int n_problems, const long *t_counts, const void *targets,
+                          const void *weights, const long *s_counts, const void *sources, void *ans);
+int fmma_plan_set_batch_graph(fmma_plan plan, int enable);   /* default 1 */
+int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable);  /* default 1 */
+/* The autotuner's choices so far: M2L tile width (boxes per workgroup) per
+ * number of parent boxes of a level.  Returns the number of entries. */
+int fmma_plan_m2l_tiles(fmma_plan plan, long *parents, int *tile, int max);
 
 /* Per-stage milliseconds, accumulated over solves like the reference's
  * time_log (include/fmma/fmma.hpp:168, src/fmm.cpp:57-141); the total is
