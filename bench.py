@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--forms", default="fp32,native,bf16x3")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
+    ap.add_argument("--batch-only", action="store_true", help="only the batching-form measurement")
+    ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -140,6 +143,70 @@ def cpu_baseline():
             "cpu": cpu_model, "omp_threads": cores}
 
 
+def batch_bench(fmma_amd, synth, torch, nprob, steps):
+    """The batching form: nprob independent 3-D Yukawa (kappa = 2) problems,
+    N log-uniform in [1024, 65536], points uniform in each problem's unit
+    cube, weights N(0,1), 5 nodes per dimension, FP32-MFMA M2L.  One batched
+    solve (replayed from its captured hipGraph, M2L tiles from the
+    autotuner) against a loop of per-problem solves on the same data (one
+    plan per Depth, buffers warmed)."""
+    c = synth.config("c5_batch", nprob)
+    counts = c["counts"]
+    x = torch.from_numpy(c["targets"]).cuda()
+    w = torch.from_numpy(c["weights"]).cuda()
+    n = x.shape[0]
+
+    def mkplan(depth=-1):
+        pl = fmma_amd.Plan(3, "float64")
+        pl.set_solver_type("fmm")
+        pl.set_poly_ord(c["poly_ord"])
+        pl.set_Depth(depth)
+        pl.set_trans_sym_flag(1)
+        pl.set_kernel(c["kernel"], *c["params"])
+        pl.set_m2l_precision("fp32")
+        return pl
+
+    pl = mkplan()
+    ans = torch.zeros(n, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    pl.solve_batch(counts, x, w, counts, x, ans)  # eager + autotuner
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    pl.solve_batch(counts, x, w, counts, x, ans)  # capture + launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pl.solve_batch(counts, x, w, counts, x, ans)
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+    print("# batch of %d problems (%d points): first call %.1f ms, graph replay %.3f ms, tiles %s"
+          % (nprob, n, first_ms, graph_ms, pl.m2l_tiles()), file=sys.stderr, flush=True)
+
+    depths = [int(np.log(k) / 3) + 1 for k in counts]
+    plans = {d: mkplan(d) for d in sorted(set(depths))}
+    ref = torch.zeros_like(ans)
+    off = np.concatenate([[0], np.cumsum(counts)])
+
+    def loop():
+        for p in range(nprob):
+            sl = slice(int(off[p]), int(off[p + 1]))
+            plans[depths[p]].solve(x[sl], w[sl], x[sl], ref[sl])
+        torch.cuda.synchronize()
+
+    loop()  # warm the buffers
+    t0 = time.perf_counter()
+    loop()
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    diff = float((ans - ref).abs().max().item())
+    print("# per-problem loop %.1f ms; max |batch - loop| %.3e" % (loop_ms, diff), file=sys.stderr, flush=True)
+    return {"workload": "%d Yukawa problems, N log-uniform in [1024, 65536], %d points, 5 nodes/dim, M2L fp32, "
+                        "targets = sources" % (nprob, n),
+            "problems": nprob, "points": int(n), "depths": {str(d): depths.count(d) for d in sorted(set(depths))},
+            "graph_replay_ms": round(graph_ms, 3), "first_call_ms": round(first_ms, 1),
+            "per_problem_loop_ms": round(loop_ms, 1), "speedup_vs_loop": round(loop_ms / graph_ms, 1),
+            "max_abs_diff_vs_loop": diff, "m2l_tiles": {str(k): v for k, v in pl.m2l_tiles().items()}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -158,6 +225,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.numel() * 4}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batching": batch_bench(fmma_amd, synth, torch, args.batch_problems, args.steps)}))
         return
     n = args.n
     x_np = synth.points(1, n, 3)
@@ -256,6 +326,10 @@ def main():
     for k in ("achieved", "peak", "frac"):
         roof[k] = float("%.4g" % roof[k])
 
+    batching = None
+    if not args.no_batch:
+        batching = batch_bench(fmma_amd, synth, torch, args.batch_problems, max(3, args.steps // 2))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -283,6 +357,7 @@ def main():
         "m2l_pairs": mpairs,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batching": batching,
     }
     if cpu and "native" in results:
         line["speedup_fp64_gpu_vs_cpu"] = round(cpu["value"] / results["native"]["ms"], 1)

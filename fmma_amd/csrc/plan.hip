@@ -135,7 +135,8 @@ struct Plan {
     DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
     DBuf t_keys, t_vals, t_keys_alt, t_vals_alt, t_hist, t_pos, t_start;
     DBuf Wm, Wl, Kraw, Kf, Kf_lo, Wc, Wc_lo, l2p_out, exact_part, p2p_slots;
-    DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks, d_op_levels, d_op_prob;
+    DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks, d_op_levels, d_op_prob, d_pairs;
+    int n_pairs = 0;
     DBuf h_t, h_s, h_w, h_ans;
 
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
@@ -252,6 +253,19 @@ static int setup_basis(Plan &pl)
         if ((rc = upload(pl.d_far, pl.offs.far.data(), pl.offs.far.size() * sizeof(int), st))) return rc;
         if ((rc = upload(pl.d_cls_f, pl.offs.cls_f.data(), pl.offs.cls_f.size() * sizeof(int), st))) return rc;
         if ((rc = upload(pl.d_cls_o, pl.offs.cls_o.data(), pl.offs.cls_o.size() * sizeof(int), st))) return rc;
+        // (o, -o) pairs of far offsets, for the paired operator build
+        std::vector<int> pairs;
+        for (int f = 0; f < pl.offs.n_far; ++f)
+            for (int g = f + 1; g < pl.offs.n_far; ++g) {
+                bool neg = true;
+                for (int d = 0; d < D; ++d) neg = neg && pl.offs.far[g * D + d] == -pl.offs.far[f * D + d];
+                if (neg) {
+                    pairs.push_back(f);
+                    pairs.push_back(g);
+                }
+            }
+        pl.n_pairs = (int)pairs.size() / 2;
+        if ((rc = upload(pl.d_pairs, pairs.data(), pairs.size() * sizeof(int), st))) return rc;
         // keep host vectors alive until the copies are done
         FMMA_HIP(hipStreamSynchronize(st));
         pl.set_n = n;
@@ -403,6 +417,8 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
             oa.batch_problem = pl.d_op_prob.as<int>();
             oa.Kf = pl.Kf.p;
             oa.Kf_lo = pl.Kf_lo.p;
+            oa.pairs = pl.d_pairs.as<int>();
+            oa.n_pairs = pl.n_pairs;
             rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st);
             if (rc == FMMA_ERR_UNSUPPORTED) {
                 // a launcher table without the fragment form: one level at a time through Kraw
@@ -956,6 +972,8 @@ static int batch_kernels(Plan &pl, BatchPlan &bp, const T *tg, const T *w, const
     oa.batch_problem = bp.d_ent_prob.as<int>();
     oa.Kf = pl.Kf.p;
     oa.Kf_lo = pl.Kf_lo.p;
+    oa.pairs = pl.d_pairs.as<int>();
+    oa.n_pairs = pl.n_pairs;
     if ((rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st)))
         return rc == FMMA_ERR_UNSUPPORTED ? fail(rc, "batch: the kernel's launcher table has no fragment form") : rc;
     if ((rc = m2l_convert<T>(Wm, bp.rows, P, oa.KP, mode, pl.Wc.p, pl.Wc_lo.p, st))) return rc;
@@ -1071,6 +1089,7 @@ static std::vector<const void *> batch_buffer_ptrs(const Plan &pl, const void *t
             pl.s_w.p, pl.s_start.p, pl.t_keys.p, pl.t_vals.p, pl.t_keys_alt.p, pl.t_vals_alt.p, pl.t_hist.p,
             pl.t_pos.p, pl.t_start.p, pl.Wm.p, pl.Wl.p, pl.Wc.p, pl.Wc_lo.p, pl.Kf.p, pl.Kf_lo.p, pl.l2p_out.p,
             pl.p2p_slots.p, pl.d_nodes.p, pl.d_coef.p, pl.d_tab.p, pl.d_far.p, pl.d_cls_f.p, pl.d_cls_o.p,
+            pl.d_pairs.p,
             b.d_toff.p, b.d_soff.p, b.d_levels.p, b.d_bbase.p, b.d_box_prob.p, b.d_geoms.p, b.d_items.p,
             b.d_tasks.p, b.d_ent_level.p, b.d_ent_prob.p, pl.ops.ctx};
 }

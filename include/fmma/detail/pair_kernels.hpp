@@ -879,33 +879,87 @@ __global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
     static_cast<T *>(a.K)[oi * PP + e] = m2l_op_value<T, D>(a, *static_cast<const T *>(a.Len), a.level, oi, kl, km, kern);
 }
 
+// Fragment index of element (row, col) of one offset's block.
+__device__ __forceinline__ long frag_index(int format, int KP, int row, int col)
+{
+    if (format == FMMA_FRAG_BF16) {
+        const int lane = (row & 15) + 16 * ((col & 31) >> 3);
+        return ((((long)(row >> 4) * (KP / 32) + (col >> 5)) * 64 + lane) << 3) + (col & 7);
+    }
+    const int lane = (row & 15) + 16 * (col & 3);
+    return (((long)(row >> 4) * (KP / 4) + (col >> 2)) << 6) + lane;
+}
+
 // Operator sets of several (problem, level) pairs straight into fragment
 // order, with the same values and roundings as m2l_op_kernel followed by the
-// library's fragment conversion.
-template <typename T, int D, class K>
+// library's fragment conversion.  Each wave covers 64 consecutive fragment
+// elements: one 16-row block of K_o and 4 columns (16x16x4 formats) or 8
+// (BF16).  Their node positions X (rows) and Y (columns) are computed once
+// per wave into LDS with exactly m2l_op_value's expressions; each thread then
+// evaluates one f(X, Y).  PAIRED: blockIdx.y is a pair (o, -o) and
+// K_{-o}[col][row] = sym * K_o[row][col] is stored from the same value
+// (translation invariance and f(y,x) = sym f(x,y)), halving the evaluations.
+template <typename T, int D, class K, bool PAIRED>
 __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K kern)
 {
-    const int oi = blockIdx.y, b = blockIdx.z;
+    __shared__ T sx[4][16][D], sy[4][8][D];
+    const int b = blockIdx.z, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int oi = PAIRED ? a.pairs[2 * blockIdx.y] : (int)blockIdx.y;
     const long per = (long)a.MP * a.KP;
-    const long r = (long)blockIdx.x * 256 + threadIdx.x;
+    const bool bf = a.format == FMMA_FRAG_BF16;
+    const int nks = bf ? a.KP / 32 : a.KP / 4;
+    const long rw = (long)blockIdx.x * 256 + wave * 64;  // the wave's first element
+    const long q = bf ? (rw >> 9) : (rw >> 6);
+    const int s = (int)(q % nks), rb = (int)(q / nks);
+    const int cbase = bf ? 32 * s + 8 * ((int)((rw >> 3) & 63) >> 4) : 4 * s;
+    const int ncol = bf ? 8 : 4;
+    const T Len = static_cast<const T *>(a.Len)[(long)a.batch_problem[b] * a.len_stride];
+    const T len = Len / (T)(1u << a.batch_level[b]);
+    if (lane < 16 + ncol) {
+        const bool isrow = lane < 16;
+        const int k = isrow ? 16 * rb + lane : cbase + lane - 16;
+        if (k < a.P) {
+            int dk = k;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+                const T c = (T)a.nodes[dk % a.n_nodes];
+                dk /= a.n_nodes;
+                if (isrow)
+                    sx[wave][lane][d] = len * (c / 2);
+                else
+                    sy[wave][lane - 16][d] = len * ((T)a.offsets[oi * D + d] + c / 2);
+            }
+        }
+    }
+    __syncthreads();
+    const long r = (long)blockIdx.x * 256 + tid;
     if (r >= per) return;
     int row, col;
     frag_rowcol(a.format, a.KP, r, row, col);
+    const bool in = row < a.P && col < a.P;
     T v = T(0);
-    if (row < a.P && col < a.P) {
-        const T Len = static_cast<const T *>(a.Len)[(long)a.batch_problem[b] * a.len_stride];
-        v = m2l_op_value<T, D>(a, Len, a.batch_level[b], oi, row, col, kern);
-    }
-    const long e = ((long)b * a.n_off + oi) * per + r;
-    if (a.format == FMMA_FRAG_F64) {
-        static_cast<double *>(a.Kf)[e] = (double)v;
-    } else if (a.format == FMMA_FRAG_F32) {
-        static_cast<float *>(a.Kf)[e] = (float)v;
-    } else {
-        const float fv = (float)v;
-        const __bf16 h = (__bf16)fv;
-        static_cast<__bf16 *>(a.Kf)[e] = h;
-        static_cast<__bf16 *>(a.Kf_lo)[e] = (__bf16)(fv - (float)h);
+    if (in) v = kern.template operator()<T, D>(sx[wave][row - 16 * rb], sy[wave][col - cbase]);
+    auto store = [&](int off, long e, T x) {
+        const long g = ((long)b * a.n_off + off) * per + e;
+        if (a.format == FMMA_FRAG_F64) {
+            static_cast<double *>(a.Kf)[g] = (double)x;
+        } else if (a.format == FMMA_FRAG_F32) {
+            static_cast<float *>(a.Kf)[g] = (float)x;
+        } else {
+            const float fv = (float)x;
+            const __bf16 h = (__bf16)fv;
+            static_cast<__bf16 *>(a.Kf)[g] = h;
+            static_cast<__bf16 *>(a.Kf_lo)[g] = (__bf16)(fv - (float)h);
+        }
+    };
+    store(oi, r, v);
+    if constexpr (PAIRED) {
+        constexpr T sym = (T)kernel_symmetry<K>::value;
+        const int ni = a.pairs[2 * blockIdx.y + 1];
+        if (in)
+            store(ni, frag_index(a.format, a.KP, col, row), sym * v);
+        else
+            store(ni, r, T(0));  // padding of K_{-o} at the same place
     }
 }
 
@@ -920,9 +974,14 @@ int launch_m2l_op(const fmma_m2l_op_args *a, const K &k, hipStream_t s)
         if (a->n_batch <= 0) return FMMA_OK;
         if (a->format == FMMA_FRAG_F64 && sizeof(T) != 8) return FMMA_ERR_INVALID;
         const long per = (long)a->MP * a->KP;
-        hipLaunchKernelGGL((m2l_op_frag_kernel<T, D, K>),
-                           dim3((unsigned)((per + 255) / 256), (unsigned)a->n_off, (unsigned)a->n_batch), dim3(256), 0,
-                           s, *a, k);
+        const unsigned nb = (unsigned)((per + 255) / 256);
+        constexpr bool sym = kernel_symmetry<K>::value == 1 || kernel_symmetry<K>::value == -1;
+        if (sym && a->pairs && 2 * a->n_pairs == a->n_off && K::translation_invariant)
+            hipLaunchKernelGGL((m2l_op_frag_kernel<T, D, K, sym>), dim3(nb, (unsigned)a->n_pairs, (unsigned)a->n_batch),
+                               dim3(256), 0, s, *a, k);
+        else
+            hipLaunchKernelGGL((m2l_op_frag_kernel<T, D, K, false>),
+                               dim3(nb, (unsigned)a->n_off, (unsigned)a->n_batch), dim3(256), 0, s, *a, k);
     }
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }

@@ -165,6 +165,11 @@ typedef struct {
     const int *batch_level;    /* [n_batch] device */
     const int *batch_problem;  /* [n_batch] device */
     void *Kf, *Kf_lo;
+    /* Optional, fragment formats of a kernel with symmetry +-1: n_pairs
+     * (offset, -offset) index pairs; only the first of each pair is
+     * evaluated, K_{-o}[km][kl] = symmetry * K_o[kl][km] is written from it. */
+    const int *pairs;          /* [n_pairs][2] device, or NULL */
+    int n_pairs;
 } fmma_m2l_op_args;
 
 typedef struct {
