@@ -265,7 +265,8 @@ def main():
 
     for form in forms:
         ms, log = run(form, args.steps, args.warmup)
-        results[form] = {"ms": round(ms, 3), "stages_ms": {k: round(v, 4) for k, v in log.items()}}
+        results[form] = {"ms": round(ms, 3), "stages_ms": {k: round(v, 4) for k, v in log.items()},
+                         "m2l_tiles": {str(k): v for k, v in plans[form].m2l_tiles(batch=False).items()}}
         print("# form %-7s %.3f ms/solve  stages %s" % (form, ms, json.dumps(results[form]["stages_ms"])),
               file=sys.stderr, flush=True)
 

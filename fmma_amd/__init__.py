@@ -251,15 +251,18 @@ class Plan:
         _lib.check(self._L.fmma_plan_set_batch_graph(self._h, 1 if on else 0))
 
     def set_m2l_autotune(self, on):
-        """Time the M2L tile widths on the first batched solve of a layout (default on)."""
+        """Time the M2L tile widths on the first solve of a Depth and form, or
+        the first batched solve of a layout (default on)."""
         _lib.check(self._L.fmma_plan_set_m2l_autotune(self._h, 1 if on else 0))
 
-    def m2l_tiles(self):
-        """{parent boxes per level: M2L tile width} chosen by the autotuner."""
-        n = self._L.fmma_plan_m2l_tiles(self._h, None, None, 0)
+    def m2l_tiles(self, batch=True):
+        """{parent boxes per level: M2L tile width} chosen by the autotuner
+        for batches (batch=True) or single solves."""
+        b = 1 if batch else 0
+        n = self._L.fmma_plan_m2l_tiles(self._h, b, None, None, 0)
         par = (ctypes.c_long * max(n, 1))()
         tb = (ctypes.c_int * max(n, 1))()
-        self._L.fmma_plan_m2l_tiles(self._h, par, tb, n)
+        self._L.fmma_plan_m2l_tiles(self._h, b, par, tb, n)
         return {int(par[i]): int(tb[i]) for i in range(n)}
 
     # ---- diagnostics

@@ -85,7 +85,7 @@ def lib():
     L.fmma_plan_solve_batch.argtypes = [vp, i, lp, vp, vp, lp, vp, vp]
     L.fmma_plan_set_batch_graph.argtypes = [vp, i]
     L.fmma_plan_set_m2l_autotune.argtypes = [vp, i]
-    L.fmma_plan_m2l_tiles.argtypes = [vp, lp, ctypes.POINTER(ctypes.c_int), i]
+    L.fmma_plan_m2l_tiles.argtypes = [vp, i, lp, ctypes.POINTER(ctypes.c_int), i]
     L.fmma_plan_solve.argtypes = [vp, l, vp, vp, l, vp, vp]
     L.fmma_plan_solve_host.argtypes = [vp, l, vp, vp, l, vp, vp]
     L.fmma_plan_time_log.argtypes = [vp, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_double), i]

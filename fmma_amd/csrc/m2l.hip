@@ -83,13 +83,15 @@ void OffsetTables::build(int dim)
 // of 256/128/64 boxes that still gives >= 512 workgroups (two per CU), when a
 // pipelined kernel exists for it; tasks of equal width form one launch.
 std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
-                                    const long *w_off, const long *c_off, const long *k_off)
+                                    const long *w_off, const long *c_off, const long *k_off, const int *tb_of_level,
+                                    int only_level)
 {
     std::vector<M2LTask> t;
     groups.clear();
     int MP, KP;
     m2l_shape(P, mode, MP, KP);
     for (int l = leaf_level; l >= 2; --l) {
+        if (only_level >= 0 && l != only_level) continue;
         const unsigned npar = 1u << (D * (l - 1));
         int tb = 64;
         for (int cand : {256, 128}) {
@@ -98,6 +100,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
                 break;
             }
         }
+        if (tb_of_level && tb_of_level[l] > 0) tb = tb_of_level[l];
         const bool piped = m2l_pipe_supported(D, KP, mode, tb);
         if (groups.empty() || groups.back().tb != tb || groups.back().pipelined != piped)
             groups.push_back(M2LGroup{tb, (int)t.size(), 0, piped});
@@ -201,6 +204,9 @@ __device__ __forceinline__ void target_coords(const M2LTask &t, int tid, int (&t
 #pragma unroll
     for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((t.cls >> (D - 1 - d)) & 1);
 }
+
+// LDS a pipelined M2L workgroup may take for two operand tiles.
+constexpr int kM2LDoubleLds = 160 * 1024 - 4096;
 
 // Operand rows staged per LDS pass of m2l_mfma4: 64 boxes x (256 + 1) FP64
 // values = 132 KB, inside the 160 KB of a CU for every P <= 512.
@@ -397,7 +403,11 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     constexpr int NPART = MODE == 2 ? 2 : 1;
     constexpr int VEC = 16 / (int)sizeof(ST);  // elements per 16-byte load
     static_assert(EPT % VEC == 0, "gather chunks must be whole 16-byte vectors");
-    __shared__ __align__(16) ST Bs[NPART * TB * LDB];
+    // Two tiles when they fit: offset i+1's tile is written while offset i's
+    // MFMAs run, one barrier per offset instead of two.
+    constexpr int BUF = NPART * TB * LDB;
+    constexpr bool DB = 2 * BUF * (int)sizeof(ST) <= kM2LDoubleLds;
+    __shared__ __align__(16) ST Bs[(DB ? 2 : 1) * BUF];
 
     // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
     // contiguous run of tasks (neighbouring tiles, all classes) so that its
@@ -471,24 +481,32 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = V4{0, 0, 0, 0};
 
     const int n = a.n_cls;
+    auto store_tile = [&](ST *tile) {  // registers -> LDS tile
+        ST *dst = tile + col * LDB + part * EPT;
+        if constexpr (MODE == 2) {
+#pragma unroll
+            for (int j = 0; j < EPT / VEC; ++j) {
+                reinterpret_cast<uint4 *>(dst)[j] = g[j];
+                reinterpret_cast<uint4 *>(dst + TB * LDB)[j] = g_lo[j];
+            }
+        } else {
+            const ST *v = reinterpret_cast<const ST *>(g);
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) dst[j] = v[j];
+        }
+    };
     gather(0);
     load_a(0);
-    for (int i = 0; i < n; ++i) {
-        {   // registers -> LDS tile of offset i
-            ST *dst = Bs + col * LDB + part * EPT;
-            if constexpr (MODE == 2) {
-#pragma unroll
-                for (int j = 0; j < EPT / VEC; ++j) {
-                    reinterpret_cast<uint4 *>(dst)[j] = g[j];
-                    reinterpret_cast<uint4 *>(dst + TB * LDB)[j] = g_lo[j];
-                }
-            } else {
-                const ST *v = reinterpret_cast<const ST *>(g);
-#pragma unroll
-                for (int j = 0; j < EPT; ++j) dst[j] = v[j];
-            }
-        }
+    if constexpr (DB) {
+        store_tile(Bs);
         __syncthreads();
+    }
+    for (int i = 0; i < n; ++i) {
+        const ST *cur = DB ? Bs + (i & 1) * BUF : Bs;
+        if constexpr (!DB) {
+            store_tile(Bs);
+            __syncthreads();
+        }
         AT acur[NKS];
         [[maybe_unused]] AT acur_lo[MODE == 2 ? NKS : 1];
 #pragma unroll
@@ -508,8 +526,8 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
                     for (int cb = 0; cb < NCB; ++cb) {
                         const int off = (16 * cb + (lane & 15)) * LDB + kk;
-                        const bf8_t bh = *reinterpret_cast<const bf8_t *>(Bs + off);
-                        const bf8_t bl = *reinterpret_cast<const bf8_t *>(Bs + TB * LDB + off);
+                        const bf8_t bh = *reinterpret_cast<const bf8_t *>(cur + off);
+                        const bf8_t bl = *reinterpret_cast<const bf8_t *>(cur + TB * LDB + off);
                         acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[s], bh, acc[cb], 0, 0, 0);
                         acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bl, acc[cb], 0, 0, 0);
                         acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bh, acc[cb], 0, 0, 0);
@@ -521,7 +539,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                     const int kk = 4 * s + (lane >> 4);
 #pragma unroll
                     for (int cb = 0; cb < NCB; ++cb) {
-                        const CT bv = Bs[(16 * cb + (lane & 15)) * LDB + kk];
+                        const CT bv = cur[(16 * cb + (lane & 15)) * LDB + kk];
                         if constexpr (MODE == 0)
                             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[s], bv, acc[cb], 0, 0, 0);
                         else
@@ -529,6 +547,9 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                     }
                 }
             }
+        }
+        if constexpr (DB) {
+            if (i + 1 < n) store_tile(Bs + ((i + 1) & 1) * BUF);
         }
         __syncthreads();
     }

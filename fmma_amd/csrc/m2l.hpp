@@ -41,10 +41,13 @@ struct M2LGroup {
     bool pipelined;
 };
 
-// Tasks of levels leaf_level .. 2 of one problem; the offsets are filled
-// from per-level tables (element offsets) given by the caller.
+// Tasks of levels leaf_level .. 2 of one problem (or of only_level >= 0);
+// the offsets are filled from per-level tables (element offsets) given by
+// the caller.  The tile width of a level is tb_of_level[l] when given and
+// positive, else the widest of 256/128/64 that keeps >= 512 workgroups.
 std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
-                                    const long *w_off, const long *c_off, const long *k_off);
+                                    const long *w_off, const long *c_off, const long *k_off,
+                                    const int *tb_of_level = nullptr, int only_level = -1);
 bool m2l_pipe_supported(int D, int KP, int mode, int tb);
 
 // Kernel arguments of the MFMA contraction.

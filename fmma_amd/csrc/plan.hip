@@ -142,7 +142,9 @@ struct Plan {
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
     std::unique_ptr<BatchPlan> batch;
     bool batch_graph = true, m2l_autotune = true;
-    std::map<long, int> tuned_tb;        // (dim, P, mode, parents per level) -> tile width
+    std::map<long, int> tuned_tb;        // batches: (dim, P, mode, parents per level) -> tile width
+    std::map<long, int> tuned_tb_single; // single problems: (dim, P, mode, parents per level) -> tile width
+    int tuned_set_L = -1, tuned_set_mode = -1;  // the single-problem tasks carry tuned widths
     hipStream_t cap_stream = nullptr;     // private stream for graph capture
 
     // cached set-up keyed by (poly_ord, leaf level, mode)
@@ -212,6 +214,11 @@ static int upload(DBuf &b, const void *src, size_t bytes, hipStream_t st)
     if (rc) return rc;
     if (bytes) FMMA_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
     return FMMA_OK;
+}
+
+static long tune_key(int D, int P, int mode, long npar)
+{
+    return (((long)D * 1024 + P) * 4 + mode) * (1L << 32) + npar;
 }
 
 static int frag_format(int mode)
@@ -322,6 +329,7 @@ static int setup(Plan &pl, int L)
         }
         pl.set_L = L;
         pl.set_mode = mode;
+        pl.tuned_set_L = -1;
     }
     return FMMA_OK;
 }
@@ -345,6 +353,67 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
     if ((rc = radix_sort(sb, n, D * L, pl.stream))) return rc;
     if ((rc = box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream))) return rc;
     return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, pl.stream);
+}
+
+// The M2L tile autotuner for one problem: per level, time every pipelined
+// tile width (16..256 target boxes per workgroup) on the level's own tasks
+// (a warm-up and 3 runs, HIP events), keep the fastest, and rebuild the
+// plan's tasks with the choices.  Runs on the first fmm solve of a (Depth,
+// form); the choices are cached by (dimension, P, form, parents per level).
+template <typename T, int D>
+static int tune_single(Plan &pl, const M2LKArgs &ka, int L)
+{
+    const int P = pl.P(), mode = pl.mode();
+    hipStream_t st = pl.stream;
+    int rc;
+    int tb_of[kMaxLevels] = {};
+    for (int l = 2; l <= L; ++l) {
+        const long npar = 1L << (D * (l - 1));
+        const long key = tune_key(D, P, mode, npar);
+        auto it = pl.tuned_tb_single.find(key);
+        if (it != pl.tuned_tb_single.end()) {
+            tb_of[l] = it->second;
+            continue;
+        }
+        float best = 1e30f;
+        int best_tb = 0;
+        for (int tb : {16, 32, 64, 128, 256}) {
+            if (!m2l_pipe_supported(D, ka.KP, mode, tb) || tb > 4 * npar) continue;
+            int one[kMaxLevels] = {};
+            one[l] = tb;
+            std::vector<M2LGroup> g;
+            std::vector<M2LTask> t = make_m2l_tasks(D, L, P, mode, g, pl.lvl_off, pl.lvlc_off, pl.kf_off, one, l);
+            DBuf dt;
+            if ((rc = upload(dt, t.data(), t.size() * sizeof(M2LTask), st))) return rc;
+            M2LKArgs a2 = ka;
+            a2.tasks = dt.as<M2LTask>();
+            if ((rc = m2l_apply<T, D>(a2, g, mode, st))) return rc;  // warm-up
+            hipEvent_t e0, e1;
+            FMMA_HIP(hipEventCreate(&e0));
+            FMMA_HIP(hipEventCreate(&e1));
+            FMMA_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < 3 && !rc; ++r) rc = m2l_apply<T, D>(a2, g, mode, st);
+            FMMA_HIP(hipEventRecord(e1, st));
+            FMMA_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FMMA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                best_tb = tb;
+            }
+        }
+        pl.tuned_tb_single[key] = best_tb;
+        tb_of[l] = best_tb;
+    }
+    std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off, tb_of);
+    pl.n_tasks = (int)tasks.size();
+    if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
+    FMMA_HIP(hipStreamSynchronize(st));
+    pl.tuned_set_L = L;
+    return FMMA_OK;
 }
 
 // Tree, P2M, M2M, M2L and L2L (stages 0-4); records pl.view.
@@ -465,6 +534,8 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
                 const long nbox = (pl.lvl_off[L + 1] - pl.lvl_off[2]) / P;
                 if ((rc = m2l_convert<T>(Wm + pl.lvl_off[2], nbox, P, ka.KP, mode, wc, wl, st))) return rc;
             }
+            if (pl.m2l_autotune && pl.tuned_set_L != L && (rc = tune_single<T, D>(pl, ka, L))) return rc;
+            ka.tasks = pl.d_tasks.as<M2LTask>();
             if ((rc = m2l_apply<T, D>(ka, pl.groups, mode, st))) return rc;
         } else {
             for (int l = 2; l <= L; ++l) {
@@ -711,11 +782,6 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
 // all problems sit in global leaf-box order (rows of P), then the coarser
 // levels of each problem; M2M/L2L run over item lists (child block, parent)
 // and M2L over tasks that carry their problem's row and operator offsets.
-
-static long tune_key(int D, int P, int mode, long npar)
-{
-    return (((long)D * 1024 + P) * 4 + mode) * (1L << 32) + npar;
-}
 
 // Tile width by rule: the narrowest pipelined width that covers a level's
 // parents, at most 64.
@@ -1369,12 +1435,12 @@ int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable)
     return FMMA_OK;
 }
 
-int fmma_plan_m2l_tiles(fmma_plan plan, long *parents, int *tile, int max)
+int fmma_plan_m2l_tiles(fmma_plan plan, int batch, long *parents, int *tile, int max)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     const Plan &pl = plan->pl;
     int i = 0;
-    for (const auto &kv : pl.tuned_tb) {
+    for (const auto &kv : batch ? pl.tuned_tb : pl.tuned_tb_single) {
         if (i < max && parents && tile) {
             parents[i] = kv.first & 0xffffffffL;
             tile[i] = kv.second;

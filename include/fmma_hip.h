@@ -324,9 +324,11 @@ int fmma_plan_solve_batch(fmma_plan plan, int n_problems, const long *t_counts, 
                           const void *weights, const long *s_counts, const void *sources, void *ans);
 int fmma_plan_set_batch_graph(fmma_plan plan, int enable);   /* default 1 */
 int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable);  /* default 1 */
-/* The autotuner's choices so far: M2L tile width (boxes per workgroup) per
- * number of parent boxes of a level.  Returns the number of entries. */
-int fmma_plan_m2l_tiles(fmma_plan plan, long *parents, int *tile, int max);
+/* The M2L tile autotuner's choices so far, for batches (batch = 1) or single
+ * solves (0): tile width (target boxes per workgroup) per number of parent
+ * boxes of a level.  Returns the number of entries.  The single-solve tuner
+ * runs on the first fmm solve of a (Depth, M2L form) when autotuning is on. */
+int fmma_plan_m2l_tiles(fmma_plan plan, int batch, long *parents, int *tile, int max);
 
 /* Per-stage milliseconds, accumulated over solves like the reference's
  * time_log (include/fmma/fmma.hpp:168, src/fmm.cpp:57-141); the total is
