@@ -23,6 +23,8 @@
  *   L2P                              src/fmm_core.cpp:580-615
  *   P2P and stage order              src/fmm.cpp:15-144
  *   exact                            src/fmma.cpp:79-94
+ *   tree (M2P per level)             src/tree.cpp:12-90, M2P src/fmm_core.cpp:625-661
+ *   nrnmm (one grid, all boxes)      src/nrnmm.cpp:13-100
  *
  * Differentiation (no counterpart in the reference; the gradient is that of
  * the approximation above): oracle_kernel_grad (each kernel's formula
@@ -762,6 +764,134 @@ static int fmm_run(int dim, int poly_ord, int depth, int trans_sym, int kid, con
     }
     for (int l = 0; l < D; ++l) { free(Wm[l]); free(Wl[l]); }
     free(Wm); free(Wl); free(lst_start); free(lst); free(T.digit);
+    return 0;
+}
+
+/* M2P of one target from box s of a level with N boxes per side
+ * (src/fmm_core.cpp:625-661): f(x, node position) * Wm[s][k] for every node,
+ * node position = (c_k + 1)/2 * len + (len * s_d + origin_d). */
+static double m2p_box(const otree_t *T, long N, int kid, const double *prm, const double *x, long sid,
+                      const double *W)
+{
+    int dim = T->dim, sc[OR_MAXD];
+    double len = T->Len / (double)N, rel[OR_MAXD], y[OR_MAXD], a = 0.0;
+    box_coords(dim, sid, N, sc);
+    for (int d = 0; d < dim; ++d) rel[d] = len * sc[d] + T->origin[d];
+    for (int k = 0; k < T->P; ++k) {
+        for (int d = 0; d < dim; ++d) y[d] = (T->c[T->digit[k * dim + d]] + 1.0) / 2.0 * len + rel[d];
+        a += oracle_kernel(kid, prm, dim, x, y) * W[sid * T->P + k];
+    }
+    return a;
+}
+
+/* The tree solver (src/tree.cpp:12-90): P2M and M2M as in fmm, then M2P from
+ * the interaction-list boxes of every level 2..Depth-1, then P2P over the
+ * leaf neighbours.  ans is overwritten (zeroed first, src/tree.cpp:64-66). */
+int oracle_tree(int dim, int poly_ord, int depth, int kid, const double *prm, long nt, const double *t,
+                long ns, const double *w, const double *s, double *ans)
+{
+    if (dim < 1 || dim > OR_MAXD || poly_ord < 0 || depth < 1) return -1;
+    if (nt == 0 || ns == 0) return 0;
+    otree_t T;
+    tree_init(&T, dim, poly_ord, depth);
+    oracle_bbox(dim, nt, t, ns, s, T.origin, &T.Len);
+    int D = depth;
+    double **Wm = (double **)calloc(D, sizeof(double *));
+    for (int l = 0; l < D; ++l) Wm[l] = (double *)calloc((size_t)ipow(1L << l, dim) * T.P, sizeof(double));
+    long *lst_start, *lst;
+    p2m(&T, ns, w, s, &lst_start, &lst, Wm[D - 1]);
+    double tab[2 * 32 * 32];
+    m2m_table(&T, tab);
+    long N = T.NL;
+    for (int step = 0; step + 3 < D; ++step) {
+        m2m(&T, N, tab, Wm[D - step - 1], Wm[D - step - 2]);
+        N /= 2;
+    }
+    double len = T.Len / (double)T.NL;
+#pragma omp parallel
+    {
+        long lst_buf[1024], nb[32];
+#pragma omp for schedule(dynamic, 64)
+        for (long i = 0; i < nt; ++i) {
+            const double *x = t + i * dim;
+            double a = 0.0;
+            long Nl = 1;
+            for (int l = 0; l < D; ++l) {
+                if (l >= 2) {
+                    int b[OR_MAXD];
+                    double ll = T.Len / (double)Nl;
+                    for (int d = 0; d < dim; ++d) b[d] = bin1(x[d], T.origin[d], ll, (int)Nl, NULL);
+                    int cnt = oracle_interaction_list(dim, b, Nl, lst_buf);
+                    for (int q = 0; q < cnt; ++q) a += m2p_box(&T, Nl, kid, prm, x, lst_buf[q], Wm[l]);
+                }
+                Nl *= 2;
+            }
+            int b[OR_MAXD];
+            for (int d = 0; d < dim; ++d) b[d] = bin1(x[d], T.origin[d], len, T.NL, NULL);
+            int cnt = oracle_neighbour_list(dim, b, T.NL, nb);
+            for (int q = 0; q < cnt; ++q)
+                for (long j = lst_start[nb[q]]; j < lst_start[nb[q] + 1]; ++j) {
+                    long sj = lst[j];
+                    a += w[sj] * oracle_kernel(kid, prm, dim, x, s + sj * dim);
+                }
+            ans[i] = a;
+        }
+    }
+    for (int l = 0; l < D; ++l) free(Wm[l]);
+    free(Wm); free(lst_start); free(lst); free(T.digit);
+    return 0;
+}
+
+/* nrn_N when it is not set: (int)(sqrt(3) * M^(1/(2 DIM))) + 1 (src/nrnmm.cpp:20-22). */
+int oracle_nrn_auto(long ns, int dim)
+{
+    return (int)(sqrt(3.0) * pow((double)ns, 1.0 / (2.0 * dim))) + 1;
+}
+
+/* The nrnmm solver (src/nrnmm.cpp:13-100): one grid of n^DIM boxes; every
+ * target takes its neighbour boxes (Chebyshev distance <= 1) directly and
+ * every other box through its multipole expansion, boxes in row-major order.
+ * ans is overwritten. */
+int oracle_nrnmm(int dim, int poly_ord, int n, int kid, const double *prm, long nt, const double *t,
+                 long ns, const double *w, const double *s, double *ans)
+{
+    if (dim < 1 || dim > OR_MAXD || poly_ord < 0) return -1;
+    if (n <= 0) n = oracle_nrn_auto(ns, dim);
+    if (nt == 0 || ns == 0) return 0;
+    otree_t T;
+    tree_init(&T, dim, poly_ord, 1);
+    T.NL = n;
+    oracle_bbox(dim, nt, t, ns, s, T.origin, &T.Len);
+    long size = ipow(n, dim);
+    double *Wm = (double *)calloc((size_t)size * T.P, sizeof(double));
+    long *lst_start, *lst;
+    p2m(&T, ns, w, s, &lst_start, &lst, Wm);
+    double len = T.Len / (double)n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < nt; ++i) {
+        const double *x = t + i * dim;
+        int tb[OR_MAXD], sb[OR_MAXD];
+        for (int d = 0; d < dim; ++d) tb[d] = bin1(x[d], T.origin[d], len, n, NULL);
+        double a = 0.0;
+        for (long sid = 0; sid < size; ++sid) {
+            box_coords(dim, sid, n, sb);
+            int far = 0;
+            for (int d = 0; d < dim; ++d) {
+                int dd = abs(sb[d] - tb[d]);
+                if (dd > far) far = dd;
+            }
+            if (far <= 1) {
+                for (long j = lst_start[sid]; j < lst_start[sid + 1]; ++j) {
+                    long sj = lst[j];
+                    a += w[sj] * oracle_kernel(kid, prm, dim, x, s + sj * dim);
+                }
+            } else {
+                a += m2p_box(&T, n, kid, prm, x, sid, Wm);
+            }
+        }
+        ans[i] = a;
+    }
+    free(Wm); free(lst_start); free(lst); free(T.digit);
     return 0;
 }
 

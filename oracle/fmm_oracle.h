@@ -38,6 +38,11 @@ int oracle_exact(int dim, int kid, const double *prm, long nt, const double *t,
 int oracle_exact_grad(int dim, int kid, const double *prm, long nt, const double *t,
                       long ns, const double *w, const double *s, double *grad);
 int oracle_auto_depth(long nt, int dim);
+int oracle_tree(int dim, int poly_ord, int depth, int kid, const double *prm, long nt, const double *t,
+                long ns, const double *w, const double *s, double *ans);
+int oracle_nrn_auto(long ns, int dim);
+int oracle_nrnmm(int dim, int poly_ord, int n, int kid, const double *prm, long nt, const double *t,
+                 long ns, const double *w, const double *s, double *ans);
 int oracle_fmm_grad(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,
                     long nt, const double *t, long ns, const double *w, const double *s, double *grad);
 int oracle_fmm(int dim, int poly_ord, int depth, int trans_sym, int kid, const double *prm,

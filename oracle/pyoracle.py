@@ -59,6 +59,11 @@ def lib():
                                        ctypes.c_long, _dp, _dp, _dp]
         L.oracle_m2m_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.oracle_kernel_grad.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_tree.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _dp,
+                                  ctypes.c_long, _dp, _dp, _dp]
+        L.oracle_nrn_auto.argtypes = [ctypes.c_long, ctypes.c_int]
+        L.oracle_nrnmm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _dp,
+                                   ctypes.c_long, _dp, _dp, _dp]
         L.oracle_cheb_dS.restype = ctypes.c_double
         L.oracle_cheb_dS.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.oracle_exact_grad.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _dp, ctypes.c_long,
@@ -87,6 +92,7 @@ def ref():
         R.ref_levels_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _dp, ctypes.c_long, _dp, ctypes.c_long, _dp, _dp, _dp, _dp]
         R.ref_set_threads.argtypes = [ctypes.c_int]
+        R.ref_set_nrn_N.argtypes = [ctypes.c_int]
         _ref = R
     return _ref
 
@@ -193,6 +199,32 @@ def fmm(targets, weights, sources, poly_ord, depth=-1, trans_sym=0, kid=0, param
     return out
 
 
+def tree(targets, weights, sources, poly_ord, depth=-1, kid=0, params=()):
+    """The oracle tree solver (M2P per level + P2P); ans overwritten."""
+    t, w, s, p = _c(targets), _c(weights), _c(sources), _prm(params)
+    dim = t.shape[1]
+    if depth <= 0:
+        depth = auto_depth(t.shape[0], dim)
+    out = np.zeros(t.shape[0])
+    if lib().oracle_tree(dim, poly_ord, depth, kid, _d(p), t.shape[0], _d(t), s.shape[0], _d(w), _d(s), _d(out)):
+        raise RuntimeError("oracle_tree failed")
+    return out
+
+
+def nrn_auto(ns, dim):
+    return lib().oracle_nrn_auto(ns, dim)
+
+
+def nrnmm(targets, weights, sources, poly_ord, nrn_N=-1, kid=0, params=()):
+    """The oracle nrnmm solver (one grid: neighbours direct, the rest by M2P)."""
+    t, w, s, p = _c(targets), _c(weights), _c(sources), _prm(params)
+    out = np.zeros(t.shape[0])
+    if lib().oracle_nrnmm(t.shape[1], poly_ord, nrn_N, kid, _d(p), t.shape[0], _d(t), s.shape[0], _d(w), _d(s),
+                          _d(out)):
+        raise RuntimeError("oracle_nrnmm failed")
+    return out
+
+
 def kernel_grad(kid, params, x, y):
     """grad_x f(x, y) of one kernel at one pair of points."""
     x, y, p = _c(x), _c(y), _prm(params)
@@ -280,11 +312,12 @@ def l2l_step(Win, dim, poly_ord, level_out):
 # ----------------------------------------------------------- the reference
 
 def ref_solve(solver, targets, weights, sources, poly_ord=1, depth=-1, trans_sym=0, kid=0, params=(),
-              ans=None, threads=0):
+              ans=None, threads=0, nrn_N=-1):
     """Run the compiled reference's FMMA<T,DIM>::solve (float32 or float64)."""
     R = ref()
     if threads:
         R.ref_set_threads(threads)
+    R.ref_set_nrn_N(int(nrn_N))
     f32 = np.asarray(targets).dtype == np.float32
     dt = np.float32 if f32 else np.float64
     t, w, s, p = _c(targets, dt), _c(weights, dt), _c(sources, dt), _prm(params)

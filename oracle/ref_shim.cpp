@@ -86,6 +86,8 @@ std::vector<std::array<T, D>> to_aos(long n, const T* p)
     return v;
 }
 
+int g_nrn_N = -1;  // set_nrn_N of the next solves (<= 0: the reference's auto rule)
+
 template <typename T, std::size_t D>
 int run(const char* solver, int poly_ord, int depth, int trans_sym, int kid, const double* prm,
         long nt, const T* t, long ns, const T* w, const T* s, T* ans)
@@ -96,6 +98,7 @@ int run(const char* solver, int poly_ord, int depth, int trans_sym, int kid, con
     f.set_poly_ord(poly_ord);
     f.set_Depth(depth);
     f.set_trans_sym_flag(trans_sym);
+    f.set_nrn_N(g_nrn_N);
     auto tv = to_aos<T, D>(nt, t), sv = to_aos<T, D>(ns, s);
     std::vector<T> wv(w, w + ns), av(ans, ans + nt);
     f.solve(tv, wv, sv, av);
@@ -188,6 +191,12 @@ int ref_levels_f64(int dim, int poly_ord, int depth, int trans_sym, int kid, con
     case 3: return Stages<double, 3>().levels(poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, Wm, Wl);
     }
     return -1;
+}
+
+int ref_set_nrn_N(int n)
+{
+    g_nrn_N = n;
+    return 0;
 }
 
 int ref_set_threads(int n)

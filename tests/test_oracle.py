@@ -318,3 +318,72 @@ def test_auto_depth_rule(oracle):
     """(int)(ln N / DIM) + 1, at least 1 (src/fmm.cpp:23-28)."""
     for n, dim in [(4096, 1), (1 << 20, 2), (1 << 22, 3), (1, 3), (2, 1)]:
         assert oracle.auto_depth(n, dim) == max(1, int(math.log(n) / dim) + 1)
+
+
+# ------------------------------------------- the tree and nrnmm solvers
+
+def test_tree_and_nrnmm_without_far_field_are_exact(oracle):
+    """Depth 2 (leaf level 1: every box a neighbour) and a grid of at most 2
+    per side have no far field: both reduce to the direct sum."""
+    t = synth.points(40, 300, 2)
+    s = synth.points(41, 250, 2)
+    w = synth.normal(42, 250)
+    ex = oracle.exact(t, w, s, 1)
+    assert rel_l2(oracle.tree(t, w, s, 3, 2, 1), ex) <= 1e-14
+    assert rel_l2(oracle.nrnmm(t, w, s, 3, 2, 1), ex) <= 1e-14
+    assert rel_l2(oracle.nrnmm(t, w, s, 3, 1, 1), ex) <= 1e-14
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_tree_depth3_is_nrnmm_on_4_per_side(oracle, dim):
+    """Tree at Depth 3 takes level 2's interaction list (every non-neighbour,
+    since all level-1 boxes touch) through M2P of leaf-level multipoles and the
+    neighbours directly: the nrnmm computation on a 4-per-side grid of the same
+    cube, in another box order."""
+    t = synth.points(43, 900, dim)
+    s = synth.points(44, 700, dim)
+    w = synth.normal(45, 700)
+    a = oracle.tree(t, w, s, 4, 3, 4, (0.2,))
+    b = oracle.nrnmm(t, w, s, 4, 4, 4, (0.2,))
+    assert rel_l2(a, b) <= 1e-13
+
+
+@pytest.mark.parametrize("dim,kid,prm", [(1, 1, ()), (2, 4, (0.3,)), (3, 1, ())])
+def test_tree_and_nrnmm_converge_to_exact_with_order(oracle, dim, kid, prm):
+    """The far field is Chebyshev interpolation of f in the source variable:
+    the error falls with the order and is tiny at order 10 for a smooth
+    kernel (a wrong node position, sign or box offset stalls it)."""
+    n = {1: 3000, 2: 2000, 3: 1200}[dim]
+    t = synth.points(46, n, dim)
+    s = synth.points(47, n, dim)
+    w = synth.uniform(48, n)
+    ex = oracle.exact(t, w, s, kid, prm)
+    for solve, size in ((oracle.tree, 4), (oracle.nrnmm, 6)):
+        ps = (2, 4, 6) if dim == 3 else (2, 4, 6, 10)
+        errs = [rel_l2(solve(t, w, s, p, size, kid, prm), ex) for p in ps]
+        assert all(b < a for a, b in zip(errs, errs[1:])), errs
+        assert errs[-1] < (1e-7 if dim == 3 else 1e-11), errs
+
+
+def test_nrn_auto_rule(oracle):
+    """(int)(sqrt(3) M^(1/(2 DIM))) + 1 (src/nrnmm.cpp:20-22)."""
+    for m, dim in [(4096, 1), (100000, 2), (1 << 20, 3), (1, 2)]:
+        assert oracle.nrn_auto(m, dim) == int(math.sqrt(3) * m ** (1.0 / (2 * dim))) + 1
+
+
+@needs_ref
+@pytest.mark.parametrize("solver,dim,n,p,size,kid,prm", [
+    ("tree", 1, 3000, 6, -1, 2, (0.1,)), ("tree", 2, 4000, 4, 5, 3, ()), ("tree", 3, 3000, 3, 4, 1, ()),
+    ("tree", 3, 5000, 2, -1, 4, (1.5,)), ("nrnmm", 1, 2000, 5, -1, 1, ()), ("nrnmm", 2, 3000, 4, 7, 3, ()),
+    ("nrnmm", 3, 2500, 3, -1, 0, ()), ("nrnmm", 3, 2000, 2, 5, 4, (1.5,))])
+def test_tree_and_nrnmm_match_compiled_reference(oracle, solver, dim, n, p, size, kid, prm):
+    t = synth.points(49, n, dim)
+    s = synth.points(50, n, dim)
+    w = synth.normal(51, n)
+    if solver == "tree":
+        a = oracle.tree(t, w, s, p, size, kid, prm)
+        b = oracle.ref_solve("tree", t, w, s, p, size, 0, kid, prm)
+    else:
+        a = oracle.nrnmm(t, w, s, p, size, kid, prm)
+        b = oracle.ref_solve("nrnmm", t, w, s, p, -1, 0, kid, prm, nrn_N=size)
+    assert rel_l2(a, b) <= 1e-13
