@@ -84,6 +84,12 @@ class Plan:
     def set_nrn_N(self, n):
         _lib.check(self._L.fmma_plan_set_nrn_N(self._h, int(n)))
 
+    @property
+    def nrn_N(self):
+        v = ctypes.c_int()
+        _lib.check(self._L.fmma_plan_get_nrn_N(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_kernel(self, name, *params):
         kid = KERNELS[name] if isinstance(name, str) else int(name)
         p = (ctypes.c_double * max(1, len(params)))(*params)

@@ -34,6 +34,7 @@ M2L_PRECISION = {"native": 0, "fp32": 1, "bf16x3": 2}
 EXPORTS = [
     "fmma_plan_create", "fmma_plan_destroy", "fmma_plan_set_solver_type", "fmma_plan_set_poly_ord",
     "fmma_plan_set_depth", "fmma_plan_get_depth", "fmma_plan_set_trans_sym_flag", "fmma_plan_set_nrn_N",
+    "fmma_plan_get_nrn_N",
     "fmma_plan_set_kernel", "fmma_plan_set_kernel_ops", "fmma_plan_set_m2l_precision", "fmma_plan_set_stream",
     "fmma_plan_set_timing", "fmma_plan_set_p2p_mutual", "fmma_plan_gradient", "fmma_plan_gradient_host",
     "fmma_plan_solve_transpose", "fmma_plan_solve_transpose_host", "fmma_plan_solve_batch",
@@ -71,6 +72,7 @@ def lib():
     L.fmma_plan_get_depth.argtypes = [vp, ctypes.POINTER(i)]
     L.fmma_plan_set_trans_sym_flag.argtypes = [vp, i]
     L.fmma_plan_set_nrn_N.argtypes = [vp, i]
+    L.fmma_plan_get_nrn_N.argtypes = [vp, ctypes.POINTER(i)]
     L.fmma_plan_set_kernel.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), i]
     L.fmma_plan_set_kernel_ops.argtypes = [vp, vp]
     L.fmma_plan_set_m2l_precision.argtypes = [vp, i]

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(P2MArgs<T> a)
 #pragma unroll
     for (int r = 0; r < KPL; ++r) acc[r] = T(0);
     const Geom<T> *gm;
-    const int NL = 1 << box_geom<T>(a, box, gm);
+    const int lv = box_geom<T>(a, box, gm), NL = a.grid_n > 0 ? a.grid_n : 1 << lv;
     const T len = gm->len_leaf;
     __syncthreads();
     for (unsigned base = s0; base < s1; base += 64) {
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
 #pragma unroll
     for (int r = 0; r < KPL; ++r) acc[r] = T(0);
     const Geom<T> *gm;
-    const int NL = 1 << box_geom<T>(a, box, gm);
+    const int lv = box_geom<T>(a, box, gm), NL = a.grid_n > 0 ? a.grid_n : 1 << lv;
     const T len = gm->len_leaf;
     __syncthreads();
     for (unsigned base = s0; base < s1; base += 64) {

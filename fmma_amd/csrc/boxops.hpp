@@ -20,6 +20,7 @@ struct P2MArgs {
     const unsigned *box_prob;
     const Geom<T> *geoms;
     const int *levels;
+    int grid_n;  // > 0: the boxes are a row-major grid_n^D grid (nrnmm), leaf_level unused
 };
 
 template <typename T>

@@ -40,6 +40,10 @@ struct builtin_ops {
     {
         return detail::launch_exact_grad<T, D, K>(a, make(ctx), (hipStream_t)s);
     }
+    static int m2p(const fmma_m2p_args *a, void *ctx, void *s)
+    {
+        return detail::launch_m2p<T, D, K>(a, make(ctx), (hipStream_t)s);
+    }
     static void fill(fmma_kernel_ops *o, void *ctx)
     {
         o->dim = D;
@@ -54,6 +58,7 @@ struct builtin_ops {
         o->p2p_grad = &p2p_grad;
         o->exact_grad = &exact_grad;
         o->transposed = nullptr;  // every built-in is symmetric or antisymmetric
+        o->m2p = &m2p;
     }
 };
 

@@ -113,6 +113,8 @@ struct BatchPlan {
 
 // "P2P sum" is the combine pass of the mutual P2P form (zero for the direct form)
 static const char *kStageNames[] = {"tree", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "P2P sum"};
+static const char *kTreeStageNames[] = {"tree", "P2M", "M2M", "M2P", "P2P"};
+static const char *kNrnStageNames[] = {"tree", "P2M", "M2P", "P2P"};
 constexpr int kStages = 8;
 
 struct Plan {
@@ -334,12 +336,22 @@ static int setup(Plan &pl, int L)
     return FMMA_OK;
 }
 
-// Sort one point set into its leaf boxes: keys -> radix sort -> ranges -> gather.
+// Sort one point set into its leaf boxes: keys -> radix sort -> ranges ->
+// gather.  Boxes: the Morton-ordered 2^L grid, or with grid_n > 0 the
+// row-major grid_n^D grid.
 template <typename T, int D>
 static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool with_w, DBuf &keys, DBuf &vals,
-                      DBuf &keys_alt, DBuf &vals_alt, DBuf &hist, DBuf &pos, DBuf *wbuf, DBuf &start)
+                      DBuf &keys_alt, DBuf &vals_alt, DBuf &hist, DBuf &pos, DBuf *wbuf, DBuf &start,
+                      int grid_n = 0)
 {
-    const unsigned nbox = 1u << (D * L);
+    unsigned nbox = 1u << (D * L);
+    int bits = D * L;
+    if (grid_n > 0) {
+        nbox = 1;
+        for (int d = 0; d < D; ++d) nbox *= (unsigned)grid_n;
+        bits = 0;
+        while (bits < 32 && (1ull << bits) < nbox) ++bits;
+    }
     int rc;
     if ((rc = keys.ensure(n * 4)) || (rc = vals.ensure(n * 4)) || (rc = keys_alt.ensure(n * 4)) ||
         (rc = vals_alt.ensure(n * 4)) || (rc = hist.ensure(radix_hist_words(n) * 4)) ||
@@ -347,10 +359,10 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
         return rc;
     if (with_w && (rc = wbuf->ensure((size_t)n * sizeof(T)))) return rc;
     const Geom<T> *g = pl.geom.as<Geom<T>>();
-    if ((rc = leaf_keys<T, D>(pts, n, g, L, keys.as<unsigned>(), pl.stream))) return rc;
+    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream))) return rc;
     SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
                    hist.as<unsigned>()};
-    if ((rc = radix_sort(sb, n, D * L, pl.stream))) return rc;
+    if ((rc = radix_sort(sb, n, bits, pl.stream))) return rc;
     if ((rc = box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream))) return rc;
     return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, pl.stream);
 }
@@ -436,7 +448,7 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     pl.mark(0);
     if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
         return rc;
-    if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), L, st))) return rc;
+    if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
     if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
                                pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start)))
         return rc;
@@ -573,6 +585,34 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     return FMMA_OK;
 }
 
+// P2P over the leaf neighbours (the tree's Morton boxes of leaf level L, or
+// with grid_n > 0 the row-major grid_n^D boxes): ans[perm[i]] = base[i] +
+// near sum (base NULL: the near sum, added onto ans when accumulate).  The
+// mutual form runs when targets are the sources and f declares its symmetry;
+// mark_mid >= 0 records that stage mark between its two launches.
+template <typename T, int D>
+static int p2p_stage(Plan &pl, int L, int grid_n, unsigned nbox, const unsigned *t_start, const T *t_pos,
+                     const unsigned *t_perm, long nt, long ns, bool shared, const T *base, T *ans, int accumulate,
+                     int mark_mid)
+{
+    hipStream_t st = pl.stream;
+    int rc;
+    const bool mutual = shared && pl.p2p_mutual && nbox > 1 && (pl.ops.symmetry == 1 || pl.ops.symmetry == -1);
+    const int n_upper = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
+    if (mutual && (rc = pl.p2p_slots.ensure((size_t)(n_upper + 1) * nt * sizeof(T)))) return rc;
+    fmma_p2p_args qa{D, pl.dtype, (long)nbox, L, t_start, pl.s_start.as<unsigned>(), t_pos, pl.s_pos.as<T>(),
+                     pl.s_w.as<T>(), nt, ns, base, t_perm, ans, accumulate,
+                     mutual ? pl.p2p_slots.p : nullptr, mutual ? pl.ops.symmetry : 0,
+                     nullptr, nullptr, nullptr, grid_n};
+    if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+    if (mark_mid >= 0) pl.mark(mark_mid);
+    if (mutual) {
+        P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), nt, base, t_perm, ans, accumulate};
+        if ((rc = p2p_combine<T, D>(ca, st))) return rc;
+    }
+    return FMMA_OK;
+}
+
 // L2P and P2P (stages 5-7) on the passes recorded in pl.view.
 template <typename T, int D>
 static int fmm_eval(Plan &pl, T *ans)
@@ -596,20 +636,8 @@ static int fmm_eval(Plan &pl, T *ans)
         base = pl.l2p_out.as<T>();
     }
     pl.mark(6);
-
-    // mutual form: the same points on both sides and a kernel that declares its symmetry
-    const bool mutual = shared && pl.p2p_mutual && L >= 1 && (pl.ops.symmetry == 1 || pl.ops.symmetry == -1);
-    const int n_upper = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
-    if (mutual && (rc = pl.p2p_slots.ensure((size_t)(n_upper + 1) * nt * sizeof(T)))) return rc;
-    fmma_p2p_args qa{D, pl.dtype, (long)nbox, L, t_start, pl.s_start.as<unsigned>(), t_pos, pl.s_pos.as<T>(),
-                     pl.s_w.as<T>(), nt, ns, base, t_perm, ans, L < 2 ? 1 : 0,
-                     mutual ? pl.p2p_slots.p : nullptr, mutual ? pl.ops.symmetry : 0};
-    if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
-    pl.mark(7);
-    if (mutual) {
-        P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), nt, base, t_perm, ans, L < 2 ? 1 : 0};
-        if ((rc = p2p_combine<T, D>(ca, st))) return rc;
-    }
+    if ((rc = p2p_stage<T, D>(pl, L, 0, nbox, t_start, t_pos, t_perm, nt, ns, shared, base, ans, L < 2 ? 1 : 0, 7)))
+        return rc;
     pl.mark(8);
     return FMMA_OK;
 }
@@ -664,6 +692,139 @@ static int run_exact(Plan &pl, long nt, const T *tg, const T *w, long ns, const 
     return pl.ops.exact(&ea, pl.ops.ctx, pl.stream);
 }
 
+// Both point sets into their leaf boxes (the sources with their weights);
+// returns the target side's sorted arrays.
+template <typename T, int D>
+static int sort_points(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, int L, int grid_n,
+                       const T *&t_pos, const unsigned *&t_start, const unsigned *&t_perm)
+{
+    int rc;
+    if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
+                               pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start, grid_n)))
+        return rc;
+    t_pos = pl.s_pos.as<T>();
+    t_start = pl.s_start.as<unsigned>();
+    t_perm = pl.s_vals.as<unsigned>();
+    if ((const void *)tg != (const void *)sr || nt != ns) {
+        if ((rc = build_tree<T, D>(pl, tg, nullptr, nt, L, false, pl.t_keys, pl.t_vals, pl.t_keys_alt,
+                                   pl.t_vals_alt, pl.t_hist, pl.t_pos, nullptr, pl.t_start, grid_n)))
+            return rc;
+        t_pos = pl.t_pos.as<T>();
+        t_start = pl.t_start.as<unsigned>();
+        t_perm = pl.t_vals.as<unsigned>();
+    }
+    return FMMA_OK;
+}
+
+// The tree solver (src/tree.cpp:12-90): P2M and M2M as in the fmm passes,
+// then M2P from the interaction lists of levels 2..L straight to the
+// targets, then P2P; ans is overwritten (src/tree.cpp:64-66).  Stage marks:
+// 0 tree, 1 P2M, 2 M2M, 3 M2P, 4 P2P, 5 end.
+template <typename T, int D>
+static int run_tree(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+{
+    pl.view.valid = false;
+    pl.last_L = -1;
+    const int L = pl.depth - 1;
+    if (D * L > 30 || L >= kMaxLevels)
+        return fail(FMMA_ERR_UNSUPPORTED, "Depth " + std::to_string(pl.depth) + " too deep for dimension " +
+                                              std::to_string(D) + " (at most 2^30 leaf boxes)");
+    const int n = pl.poly_ord + 1, P = pl.P();
+    if (P > kMaxP) return fail(FMMA_ERR_UNSUPPORTED, "(poly_ord+1)^DIM > 512 is not supported");
+    int rc;
+    if ((rc = setup_basis(pl))) return rc;
+    long off[kMaxLevels + 1], o = 0;
+    for (int l = 0; l <= L; ++l) {
+        off[l] = o;
+        o += (long)P << (D * l);
+    }
+    if ((rc = pl.Wm.ensure(o * pl.tsize()))) return rc;
+    hipStream_t st = pl.stream;
+    const bool shared = (const void *)tg == (const void *)sr && nt == ns;
+    const unsigned nbox = 1u << (D * L);
+
+    pl.mark(0);
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+        return rc;
+    if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
+    const T *t_pos;
+    const unsigned *t_start, *t_perm;
+    if ((rc = sort_points<T, D>(pl, nt, tg, w, ns, sr, L, 0, t_pos, t_start, t_perm))) return rc;
+    pl.mark(1);
+
+    T *Wm = pl.Wm.as<T>();
+    P2MArgs<T> pa{pl.s_start.as<unsigned>(), pl.s_pos.as<T>(), pl.s_w.as<T>(), ns, pl.geom.as<Geom<T>>(), L,
+                  pl.d_coef.as<T>(), n, P, Wm + off[L]};
+    if ((rc = p2m<T, D>(pa, nbox, st))) return rc;
+    pl.mark(2);
+    for (int l = L - 1; l >= 2; --l)
+        if ((rc = m2m<T, D>(Wm + off[l + 1], Wm + off[l], 1L << (D * l), pl.d_tab.as<T>(), n, P, st))) return rc;
+    pl.mark(3);
+
+    const T *base = nullptr;
+    if (L >= 2) {
+        if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
+        fmma_m2p_args ma{D, pl.dtype, n, P, pl.d_nodes.as<double>(), (long)nbox, t_start, t_pos, nt,
+                         pl.geom.as<Geom<T>>()->origin, &pl.geom.as<Geom<T>>()->Len, L, 0, Wm, pl.l2p_out.p};
+        if ((rc = pl.ops.m2p(&ma, pl.ops.ctx, st))) return rc;
+        base = pl.l2p_out.as<T>();
+    }
+    pl.mark(4);
+    if ((rc = p2p_stage<T, D>(pl, L, 0, nbox, t_start, t_pos, t_perm, nt, ns, shared, base, ans, 0, -1))) return rc;
+    pl.mark(5);
+    return FMMA_OK;
+}
+
+// The nrnmm solver (src/nrnmm.cpp:13-100): one grid of nrn_N^DIM boxes,
+// row-major; P2M on it, M2P from every box at Chebyshev distance >= 2 and
+// P2P from the others; ans is overwritten.  Stage marks: 0 tree, 1 P2M,
+// 2 M2P, 3 P2P, 4 end.
+template <typename T, int D>
+static int run_nrnmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const T *sr, T *ans)
+{
+    pl.view.valid = false;
+    pl.last_L = -1;
+    const int g = pl.nrn_N;
+    long nb = 1;
+    for (int d = 0; d < D; ++d) nb *= g;
+    if (g < 1 || nb > (1L << 30))
+        return fail(FMMA_ERR_UNSUPPORTED, "nrn_N " + std::to_string(g) + " gives more than 2^30 boxes");
+    const unsigned nbox = (unsigned)nb;
+    const int n = pl.poly_ord + 1, P = pl.P();
+    if (P > kMaxP) return fail(FMMA_ERR_UNSUPPORTED, "(poly_ord+1)^DIM > 512 is not supported");
+    int rc;
+    if ((rc = setup_basis(pl))) return rc;
+    if ((rc = pl.Wm.ensure((size_t)nb * P * pl.tsize()))) return rc;
+    hipStream_t st = pl.stream;
+    const bool shared = (const void *)tg == (const void *)sr && nt == ns;
+
+    pl.mark(0);
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+        return rc;
+    if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), g, st))) return rc;
+    const T *t_pos;
+    const unsigned *t_start, *t_perm;
+    if ((rc = sort_points<T, D>(pl, nt, tg, w, ns, sr, 0, g, t_pos, t_start, t_perm))) return rc;
+    pl.mark(1);
+
+    T *Wm = pl.Wm.as<T>();
+    P2MArgs<T> pa{pl.s_start.as<unsigned>(), pl.s_pos.as<T>(), pl.s_w.as<T>(), ns, pl.geom.as<Geom<T>>(), 0,
+                  pl.d_coef.as<T>(), n, P, Wm, nullptr, nullptr, nullptr, g};
+    if ((rc = p2m<T, D>(pa, nbox, st))) return rc;
+    pl.mark(2);
+
+    if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
+    fmma_m2p_args ma{D, pl.dtype, n, P, pl.d_nodes.as<double>(), (long)nbox, t_start, t_pos, nt,
+                     pl.geom.as<Geom<T>>()->origin, &pl.geom.as<Geom<T>>()->Len, 0, g, Wm, pl.l2p_out.p};
+    if ((rc = pl.ops.m2p(&ma, pl.ops.ctx, st))) return rc;
+    pl.mark(3);
+    if ((rc = p2p_stage<T, D>(pl, 0, g, nbox, t_start, t_pos, t_perm, nt, ns, shared, pl.l2p_out.as<T>(), ans, 0,
+                              -1)))
+        return rc;
+    pl.mark(4);
+    return FMMA_OK;
+}
+
 enum class Op { Solve, Gradient, Transpose };
 
 // out = the solve's ans [n_t], the gradient [n_t][D], or the transpose [n_s].
@@ -674,8 +835,11 @@ static int op_typed(Plan &pl, Op op, long nt, const void *tg_, const void *w_, l
     const T *tg = (const T *)tg_, *w = (const T *)w_, *sr = (const T *)sr_;
     T *out = (T *)out_;
     int rc;
-    if (op == Op::Solve)
+    if (op == Op::Solve) {
+        if (pl.solver == "tree") return run_tree<T, D>(pl, nt, tg, w, ns, sr, out);
+        if (pl.solver == "nrnmm") return run_nrnmm<T, D>(pl, nt, tg, w, ns, sr, out);
         return is_fmm ? run_fmm<T, D>(pl, nt, tg, w, ns, sr, out) : run_exact<T, D>(pl, nt, tg, w, ns, sr, out);
+    }
     if (op == Op::Gradient) {
         if (!is_fmm) {
             fmma_exact_grad_args ea{D, pl.dtype, nt, ns, tg, w, sr, out};
@@ -707,14 +871,20 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
                   bool reuse)
 {
     const std::string &s = pl.solver;
-    const bool is_fmm = s == "fmm";
-    if (!is_fmm && s != "exact" && s != "exact_matvec") {
-        if (s == "tree" || s == "nrnmm")
-            return fail(FMMA_ERR_UNSUPPORTED, "solver type " + s + " does not run on the GPU");
+    const bool is_fmm = s == "fmm", is_tree = s == "tree", is_nrnmm = s == "nrnmm";
+    if (!is_fmm && !is_tree && !is_nrnmm && s != "exact" && s != "exact_matvec")
         return fail(FMMA_ERR_SOLVER, "solve type " + s + " not undefined");
-    }
+    if ((is_tree || is_nrnmm) && op != Op::Solve)
+        return fail(FMMA_ERR_UNSUPPORTED, "the gradient and the transpose run with the fmm and exact solvers");
+    if ((is_tree || is_nrnmm) && !pl.ops.m2p)
+        return fail(FMMA_ERR_UNSUPPORTED, "the kernel's launcher table has no M2P (the tree and nrnmm solvers)");
     if (nt < 0 || ns < 0) return fail(FMMA_ERR_INVALID, "negative size");
-    if (is_fmm && pl.depth <= 0) {
+    if (is_nrnmm && pl.nrn_N <= 0) {
+        // (int)(sqrt(3) M^(1/(2 DIM))) + 1 from the source count in TYPE, kept afterwards (src/nrnmm.cpp:20-22)
+        const double m = pl.dtype == FMMA_F64 ? (double)ns : (double)(float)ns;
+        pl.nrn_N = (int)((int)(std::sqrt(3.0) * std::pow(m, 1.0 / (2.0 * pl.dim))) + 1.0);
+    }
+    if ((is_fmm || is_tree) && pl.depth <= 0) {
         // auto Depth from the target count, kept afterwards (src/fmm.cpp:23-28); the
         // transpose and the gradient use the Depth of the solve they belong to
         pl.depth = (int)(std::log((double)nt) / pl.dim) + 1;
@@ -741,7 +911,10 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
         for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
         pl.events_ready = true;
     }
-    const bool staged = is_fmm && op == Op::Solve;  // the fmm solve records its own stage marks
+    // the fmm, tree and nrnmm solves record their own stage marks
+    const bool staged = (is_fmm || is_tree || is_nrnmm) && op == Op::Solve;
+    const char *const *names = is_tree ? kTreeStageNames : is_nrnmm ? kNrnStageNames : kStageNames;
+    const int nst = !staged ? kStages : is_tree ? 5 : is_nrnmm ? 4 : kStages;
     if (!staged) pl.mark(0);
     int rc;
     if (pl.dtype == FMMA_F64) {
@@ -758,12 +931,12 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
     FMMA_HIP(hipStreamSynchronize(pl.stream));
     if (pl.timing && pl.events_ready) {
         float total = 0.f;
-        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
+        FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[nst]));
         if (staged) {
-            for (int i = 0; i < kStages; ++i) {
+            for (int i = 0; i < nst; ++i) {
                 float ms = 0.f;
                 FMMA_HIP(hipEventElapsedTime(&ms, pl.ev[i], pl.ev[i + 1]));
-                pl.time_log[kStageNames[i]] += ms;
+                pl.time_log[names[i]] += ms;
             }
         }
         const char *what = op == Op::Solve ? "" : op == Op::Gradient ? " gradient" : " transpose";
@@ -1353,6 +1526,13 @@ int fmma_plan_set_nrn_N(fmma_plan plan, int nrn_N)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.nrn_N = nrn_N;
+    return FMMA_OK;
+}
+
+int fmma_plan_get_nrn_N(fmma_plan plan, int *nrn_N)
+{
+    if (!plan || !nrn_N) return fmma::lib::fail(FMMA_ERR_INVALID, "null pointer");
+    *nrn_N = plan->pl.nrn_N;
     return FMMA_OK;
 }
 

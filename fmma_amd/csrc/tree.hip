@@ -67,9 +67,10 @@ __global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T
 }
 
 // Len = max_d (max_d - min_d), origin_d = (max_d + min_d)/2 - Len/2, in TYPE
-// (src/fmm_core.cpp:164-171); len_leaf = Len / (TYPE)N_L.
+// (src/fmm_core.cpp:164-171); len_leaf = Len / (TYPE)n_side, n_side the leaf
+// boxes per side (2^leaf_level; nrnmm: nrn_N, src/nrnmm.cpp:37).
 template <typename T, int D>
-__global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> *g, int leaf_level)
+__global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> *g, int n_side)
 {
     T lo[D], hi[D];
 #pragma unroll
@@ -103,19 +104,19 @@ __global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> 
         for (int d = 0; d < D; ++d) g->origin[d] = (hi[d] + lo[d]) / 2 - Len / 2;
         for (int d = D; d < 3; ++d) g->origin[d] = T(0);
         g->Len = Len;
-        g->len_leaf = Len / (T)(1u << leaf_level);
+        g->len_leaf = Len / (T)n_side;
     }
 }
 
 template <typename T, int D>
-int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int leaf_level, hipStream_t st)
+int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_side, hipStream_t st)
 {
     const long n = nt > ns ? nt : ns;
     int nb = (int)((n + 255) / 256);
     nb = nb < kBBoxParts ? nb : kBBoxParts;
     nb = nb < 1 ? 1 : nb;
     hipLaunchKernelGGL((bbox_partial<T, D>), dim3(nb), dim3(256), 0, st, t, nt, s, ns, part);
-    hipLaunchKernelGGL((bbox_final<T, D>), dim3(1), dim3(64), 0, st, part, nb, g, leaf_level);
+    hipLaunchKernelGGL((bbox_final<T, D>), dim3(1), dim3(64), 0, st, part, nb, g, n_side);
     FMMA_LAUNCHED("bbox");
     return FMMA_OK;
 }
@@ -124,24 +125,31 @@ int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int leaf
 
 template <typename T, int D>
 __global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
-                                                   unsigned *keys)
+                                                   int grid_n, unsigned *keys)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int NL = 1 << leaf_level;
+    const int NL = grid_n > 0 ? grid_n : 1 << leaf_level;
     const T len = g->len_leaf;
     int b[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) b[d] = leaf_coord(pts[i * D + d], g->origin[d], len, NL);
-    keys[i] = morton_encode<D>(b);
+    if (grid_n > 0) {
+        unsigned id = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) id = id * (unsigned)grid_n + (unsigned)b[d];
+        keys[i] = id;
+    } else {
+        keys[i] = morton_encode<D>(b);
+    }
 }
 
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, unsigned *keys, hipStream_t st)
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st)
 {
     if (n <= 0) return FMMA_OK;
     hipLaunchKernelGGL((keys_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, n, g,
-                       leaf_level, keys);
+                       leaf_level, grid_n, keys);
     FMMA_LAUNCHED("leaf_keys");
     return FMMA_OK;
 }
@@ -396,7 +404,7 @@ int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws
 
 #define FMMA_TREE_INST(T, D)                                                                                    \
     template int bbox<T, D>(const T *, long, const T *, long, T *, Geom<T> *, int, hipStream_t);                \
-    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, unsigned *, hipStream_t);               \
+    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t);          \
     template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);
 FMMA_TREE_INST(double, 1)
 FMMA_TREE_INST(double, 2)

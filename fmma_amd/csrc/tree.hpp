@@ -15,11 +15,14 @@ struct SortBuffers {
     unsigned *hist;      // [radix_hist_words(n)]
 };
 
+// The bounding cube of targets and sources; len_leaf = Len / n_side.
 template <typename T, int D>
-int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int leaf_level, hipStream_t st);
+int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_side, hipStream_t st);
 
+// Leaf box of each point: Morton key on the 2^leaf_level grid, or with
+// grid_n > 0 the row-major index on a grid_n^D grid.
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, unsigned *keys, hipStream_t st);
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st);
 
 // Stable sort of b.keys (low `bits` bits significant); b.vals gets the
 // permutation.  bits == 0 only fills b.vals with 0..n-1.

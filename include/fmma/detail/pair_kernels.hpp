@@ -13,6 +13,7 @@
 //   exact        src/fmma.cpp:79-94
 //   M2L operator src/fmm_core.cpp:388-418 (translation-symmetric branch)
 //   M2L general  src/fmm_core.cpp:456-492 (kernel evaluated per box pair)
+//   M2P          src/fmm_core.cpp:625-661, src/nrnmm.cpp:79-91 (tree, nrnmm)
 // and, new, the near field and all-pairs forms of the gradient with respect
 // to the target coordinates.
 #pragma once
@@ -108,6 +109,38 @@ __device__ __forceinline__ int leaf_grid(const A &a, unsigned box, unsigned &gba
     return a.leaf_level;
 }
 
+// Grid coordinates of a leaf box, and the boxes per side: a Morton tree (or
+// a batch's problem tree), or a row-major grid of grid_n per side (nrnmm).
+template <int D, class A>
+__device__ __forceinline__ int box_coords_of(const A &a, unsigned box, int (&bc)[D], unsigned &gbase)
+{
+    if (a.grid_n > 0) {
+        gbase = 0;
+        unsigned r = box;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            bc[d] = (int)(r % (unsigned)a.grid_n);
+            r /= (unsigned)a.grid_n;
+        }
+        return a.grid_n;
+    }
+    const int L = leaf_grid(a, box, gbase);
+    morton_decode<D>(box - gbase, bc);
+    return 1 << L;
+}
+
+template <int D, class A>
+__device__ __forceinline__ unsigned box_index_of(const A &a, const int (&m)[D], unsigned gbase)
+{
+    if (a.grid_n > 0) {
+        unsigned id = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) id = id * (unsigned)a.grid_n + (unsigned)m[d];
+        return id;
+    }
+    return gbase + morton_encode<D>(m);
+}
+
 // ----------------------------------------------------------------- P2P
 //
 // One wave (64 threads) per target leaf box (Morton order).  Lane l is
@@ -180,8 +213,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
     if (lane < (unsigned)NNB) {
         int bc[D], m[D];
         unsigned gbase;
-        const int NL = 1 << leaf_grid(a, box, gbase);
-        morton_decode<D>(box - gbase, bc);
+        const int NL = box_coords_of<D>(a, box, bc, gbase);
         bool ok = true;
         constexpr int centre = (NNB - 1) / 2;
         int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
@@ -193,7 +225,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
         }
         unsigned first = 0;
         if (ok) {
-            const unsigned mm = gbase + morton_encode<D>(m);
+            const unsigned mm = box_index_of<D>(a, m, gbase);
             first = a.s_start[mm];
             cnt_nb = a.s_start[mm + 1] - first;
         }
@@ -426,8 +458,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
         const bool upper = lane <= (unsigned)NU;
         int bc[D], m[D];
         unsigned gbase;
-        const int NL = 1 << leaf_grid(a, box, gbase);
-        morton_decode<D>(box - gbase, bc);
+        const int NL = box_coords_of<D>(a, box, bc, gbase);
         bool ok = true;
         int k = lane == 0 ? NU : upper ? NU + (int)lane : (int)lane;
         const int sgn = upper ? 1 : -1;
@@ -439,7 +470,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
         }
         unsigned first = 0, cnt = 0;
         if (ok) {
-            const unsigned mm = gbase + morton_encode<D>(m);
+            const unsigned mm = box_index_of<D>(a, m, gbase);
             first = a.s_start[mm];
             cnt = a.s_start[mm + 1] - first;
         }
@@ -625,8 +656,7 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
     if (lane < (unsigned)NNB) {
         int bc[D], m[D];
         unsigned gbase;
-        const int NL = 1 << leaf_grid(a, box, gbase);
-        morton_decode<D>(box - gbase, bc);
+        const int NL = box_coords_of<D>(a, box, bc, gbase);
         bool ok = true;
         constexpr int centre = (NNB - 1) / 2;
         int k = lane == 0 ? centre : (int)lane <= centre ? (int)lane - 1 : (int)lane;
@@ -638,7 +668,7 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
         }
         unsigned first = 0;
         if (ok) {
-            const unsigned mm = gbase + morton_encode<D>(m);
+            const unsigned mm = box_index_of<D>(a, m, gbase);
             first = a.s_start[mm];
             cnt_nb = a.s_start[mm + 1] - first;
         }
@@ -1060,6 +1090,204 @@ int launch_m2l_general(const fmma_m2l_general_args *a, const K &k, hipStream_t s
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
 }
 
+// ----------------------------------------------------------------- M2P
+//
+// The far field of the tree and nrnmm solvers.  One wave per target box
+// holds the box's targets in registers as the direct P2P does (slot/slice
+// lanes, at most kP2PTargets per lane).  Far boxes are enumerated 64
+// candidates at a time -- tree: for each level 2..leaf_level the 6^D children
+// of the 3^D block around the parent of the target's box at that level;
+// nrnmm: the whole grid, row-major -- and those at Chebyshev distance >= 2
+// are compacted (ballot + popcount) into an LDS list of (corner, side,
+// multipole offset).  The list's (node position, multipole) records are then
+// staged through LDS in blocks of kP2PBlock and taken by every target, with
+// f's far() form (a node of a far box never coincides with a target).
+constexpr int kM2PList = 192;  // far boxes per list; flushed before another 64 could overflow it
+constexpr int kM2PMaxNodes = 16;
+
+template <typename T, int D, class K>
+__device__ __forceinline__ void p2p_block_far(int tpl, const T *src, unsigned cnt, unsigned q, unsigned nslice,
+                                              const T (&x)[kP2PTargets][D], T (&acc)[kP2PTargets], const K &kern)
+{
+    switch (tpl) {
+    case 1: p2p_range<1, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 2: p2p_range<2, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 3: p2p_range<3, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 4: p2p_range<4, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 5: p2p_range<5, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 6: p2p_range<6, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 7: p2p_range<7, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 8: p2p_range<8, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 9: p2p_range<9, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 10: p2p_range<10, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    case 11: p2p_range<11, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    default: p2p_range<12, true>(src, 0, cnt, q, nslice, x, acc, kern); break;
+    }
+}
+
+template <typename T, int D, class K>
+__global__ __launch_bounds__(64) void m2p_kernel(fmma_m2p_args a, K kern)
+{
+    constexpr int RW = p2p_rec<D>::W;
+    constexpr int C6 = D == 1 ? 6 : D == 2 ? 36 : 216;  // candidates per tree level
+    __shared__ __align__(16) T src[kP2PBlock * RW];
+    __shared__ T ent_rel[kM2PList * D];
+    __shared__ T ent_len[kM2PList];
+    __shared__ long ent_off[kM2PList];
+    __shared__ double hc[kM2PMaxNodes];
+
+    const unsigned box = blockIdx.x;
+    const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
+    if (t0 == t1) return;
+    const unsigned lane = threadIdx.x;
+    const unsigned nn = (unsigned)a.n_nodes, P = (unsigned)a.P;
+    // (c_j + 1)/2 of the TYPE node, in double as the reference forms it
+    if (lane < nn) hc[lane] = ((double)(T)a.nodes[lane] + 1.0) / 2.0;
+    const T *tp = static_cast<const T *>(a.t_pos);
+    const T *Wm = static_cast<const T *>(a.Wm);
+    const T *org = static_cast<const T *>(a.origin);
+    const T Len = *static_cast<const T *>(a.Len);
+    const long nt = a.n_t;
+    const int gn = a.grid_n, L = a.leaf_level;
+
+    int bc[D];
+    long ncand;
+    if (gn > 0) {
+        unsigned r = box;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            bc[d] = (int)(r % (unsigned)gn);
+            r /= (unsigned)gn;
+        }
+        ncand = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) ncand *= gn;
+    } else {
+        morton_decode<D>(box, bc);
+        ncand = L >= 2 ? (long)(L - 1) * C6 : 0;
+    }
+
+    const unsigned ntb = t1 - t0;
+    unsigned nslot = 4;
+    while (nslot < 64 && nslot * kP2PTargets < ntb) nslot <<= 1;
+    const unsigned nslice = 64 / nslot, s = lane % nslot, q = lane / nslot;
+
+    for (unsigned pass0 = t0; pass0 < t1; pass0 += nslot * kP2PTargets) {
+        const unsigned npass = t1 - pass0 < nslot * kP2PTargets ? t1 - pass0 : nslot * kP2PTargets;
+        const int tpl = (int)((npass + nslot - 1) / nslot);
+        T x[kP2PTargets][D], acc[kP2PTargets];
+#pragma unroll
+        for (int k = 0; k < kP2PTargets; ++k) {
+            acc[k] = T(0);
+            const unsigned i = pass0 + s + nslot * k;
+            const unsigned ic = i < t1 ? i : t1 - 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
+        }
+        int list_n = 0;
+        for (long c0 = 0; c0 < ncand; c0 += 64) {
+            const long c = c0 + lane;
+            bool ok = false;
+            int m[D], ref[D], side = 1;
+            long lbase = 0;
+            if (c < ncand) {
+                if (gn > 0) {
+                    long r = c;
+#pragma unroll
+                    for (int d = D - 1; d >= 0; --d) {
+                        m[d] = (int)(r % gn);
+                        r /= gn;
+                        ref[d] = bc[d];
+                    }
+                    side = gn;
+                } else {
+                    const int l = 2 + (int)(c / C6);
+                    int j = (int)(c - (long)(l - 2) * C6);
+#pragma unroll
+                    for (int d = D - 1; d >= 0; --d) {
+                        ref[d] = bc[d] >> (L - l);
+                        m[d] = 2 * ((ref[d] >> 1) - 1) + j % 6;
+                        j /= 6;
+                    }
+                    side = 1 << l;
+                    lbase = (long)P * (((1L << (D * l)) - 1) / ((1L << D) - 1));
+                }
+                int far = 0;
+                ok = true;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    ok = ok && m[d] >= 0 && m[d] < side;
+                    const int dd = m[d] > ref[d] ? m[d] - ref[d] : ref[d] - m[d];
+                    far = dd > far ? dd : far;
+                }
+                ok = ok && far >= 2;
+            }
+            const unsigned long long mask = __ballot(ok);
+            if (ok) {
+#pragma clang fp contract(off)
+                const int pos = list_n + (int)__popcll(mask & ((1ull << lane) - 1ull));
+                const T len = Len / (T)side;
+#pragma unroll
+                for (int d = 0; d < D; ++d) ent_rel[pos * D + d] = len * (T)m[d] + org[d];
+                ent_len[pos] = len;
+                ent_off[pos] = gn > 0 ? c * (long)P : lbase + (long)morton_encode<D>(m) * P;
+            }
+            list_n += (int)__popcll(mask);
+            if (list_n > kM2PList - 64 || c0 + 64 >= ncand) {
+                const long nrec = (long)list_n * P;
+                for (long rb = 0; rb < nrec; rb += kP2PBlock) {
+                    const unsigned cnt = nrec - rb < kP2PBlock ? (unsigned)(nrec - rb) : (unsigned)kP2PBlock;
+                    __syncthreads();  // list written, previous block consumed
+                    for (unsigned e = lane; e < cnt; e += 64) {
+#pragma clang fp contract(off)
+                        const unsigned p = (unsigned)rb + e;
+                        const unsigned ent = p / P, k = p - ent * P;
+                        const double ln = (double)ent_len[ent];
+                        T *r = src + e * RW;
+                        unsigned kk = k;
+#pragma unroll
+                        for (int d = D - 1; d >= 0; --d) {
+                            const unsigned dg = kk % nn;
+                            kk /= nn;
+                            r[d] = (T)(hc[dg] * ln + (double)ent_rel[ent * D + d]);
+                        }
+                        r[D] = Wm[ent_off[ent] + k];
+                    }
+                    __syncthreads();
+                    p2p_block_far<T, D>(tpl, src, cnt, q, nslice, x, acc, kern);
+                }
+                __syncthreads();  // the list is rewritten next
+                list_n = 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kP2PTargets; ++k) {
+            if (k < tpl) {
+                T v = acc[k];
+                for (unsigned o = nslot; o < 64; o <<= 1) v += __shfl_xor(v, (int)o);
+                acc[k] = v;
+            }
+        }
+        if (q == 0) {
+            T *out = static_cast<T *>(a.out);
+#pragma unroll
+            for (int k = 0; k < kP2PTargets; ++k) {
+                const unsigned i = pass0 + s + nslot * k;
+                if (k < tpl && i < t1) out[i] = acc[k];
+            }
+        }
+    }
+}
+
+template <typename T, int D, class K>
+int launch_m2p(const fmma_m2p_args *a, const K &k, hipStream_t s)
+{
+    if (a->n_target_boxes <= 0 || a->n_t <= 0) return FMMA_OK;
+    if (a->n_nodes < 1 || a->n_nodes > kM2PMaxNodes) return FMMA_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((m2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+    return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
+}
+
 // ------------------------------------------------- launcher table builder
 //
 // Fills an fmma_kernel_ops for functor K whose state lives at `ctx`
@@ -1090,6 +1318,10 @@ struct ops_for {
     {
         return launch_exact_grad<T, D, K>(a, *static_cast<const K *>(ctx), (hipStream_t)s);
     }
+    static int m2p(const fmma_m2p_args *a, void *ctx, void *s)
+    {
+        return launch_m2p<T, D, K>(a, *static_cast<const K *>(ctx), (hipStream_t)s);
+    }
     static fmma_kernel_ops make(const K *ctx)
     {
         fmma_kernel_ops o;
@@ -1105,6 +1337,7 @@ struct ops_for {
         o.p2p_grad = has_grad<K, T, D>::value ? &p2p_grad : nullptr;
         o.exact_grad = has_grad<K, T, D>::value ? &exact_grad : nullptr;
         o.transposed = nullptr;
+        o.m2p = &m2p;
         return o;
     }
 };

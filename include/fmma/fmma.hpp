@@ -13,7 +13,6 @@
 //    (set_fn, from fmma/fmma_device.hpp, compiled with hipcc) or one of the
 //    built-in kernels (set_kernel); a host-only std::function cannot run on
 //    the GPU and is rejected at compile time.
-//  * "tree" and "nrnmm" throw std::runtime_error (not built for the GPU).
 //  * New setters: set_kernel, set_m2l_precision; new getter get_time_log;
 //    new differentiation entry points gradient and solve_transpose.
 // Misuse the reference ends with exit(EXIT_FAILURE) ends the same way here:

@@ -97,6 +97,9 @@ typedef struct {
     const unsigned *box_problem;
     const unsigned *problem_box_base;
     const int *problem_leaf_level;
+    /* > 0: the leaf boxes are a row-major grid of grid_n per side (the nrnmm
+     * solver) instead of a Morton-ordered 2^leaf_level tree. */
+    int grid_n;
 } fmma_p2p_args;
 
 typedef struct {
@@ -130,7 +133,36 @@ typedef struct {
     const unsigned *box_problem;  /* a batch, as in fmma_p2p_args (else NULL) */
     const unsigned *problem_box_base;
     const int *problem_leaf_level;
+    int grid_n;                   /* as in fmma_p2p_args */
 } fmma_p2p_grad_args;
+
+/* Far field of the tree and nrnmm solvers (M2P, src/fmm_core.cpp:625-661 and
+ * src/nrnmm.cpp:79-91): out[i] = sum over the far boxes s of target i and
+ * their nodes k of f(x_i, y_{s,k}) Wm[s][k], node position
+ * y_{s,k,d} = (c_{k_d} + 1)/2 len + (len s_d + origin_d), len = Len / (boxes
+ * per side of s's level), k = sum_d k_d n^(dim-1-d).
+ *  grid_n == 0 (tree): the target boxes are the Morton-ordered 2^leaf_level
+ *  grid; target i's far boxes are the interaction lists of its box at every
+ *  level l = 2..leaf_level (the children of the parent's neighbours that are
+ *  not neighbours, src/fmm_core.cpp:15-63), whose multipoles sit in Morton
+ *  order at Wm + P (2^(dim l) - 1)/(2^dim - 1).
+ *  grid_n > 0 (nrnmm): the target boxes are the row-major grid_n^dim grid and
+ *  the far boxes every box of it at Chebyshev distance >= 2, multipoles
+ *  [grid_n^dim][P] row-major. */
+typedef struct {
+    int dim, dtype;
+    int n_nodes, P;
+    const double *nodes;       /* [n_nodes] Chebyshev nodes, device */
+    long n_target_boxes;
+    const unsigned *t_start;   /* [n_target_boxes+1] target ranges in sorted order */
+    const void *t_pos;         /* [dim][n_t] sorted target coordinates */
+    long n_t;
+    const void *origin;        /* [dim] TYPE, device: the bounding cube's corner */
+    const void *Len;           /* TYPE, device: its side */
+    int leaf_level, grid_n;
+    const void *Wm;
+    void *out;                 /* [n_t] sorted order, overwritten */
+} fmma_m2p_args;
 
 /* All-pairs gradient: grad[i][d] = sum_j w_j d/dx_d f(x_i, y_j). */
 typedef struct {
@@ -202,6 +234,8 @@ typedef struct fmma_kernel_ops_s {
     /* The table of f^T(x,y) = f(y,x), for fmma_plan_solve_transpose when
      * symmetry == 0 (NULL otherwise: f^T is then symmetry * f). */
     const struct fmma_kernel_ops_s *transposed;
+    /* Optional (NULL: the tree and nrnmm solvers are unavailable): M2P. */
+    int (*m2p)(const fmma_m2p_args *, void *ctx, void *stream);
 } fmma_kernel_ops;
 
 /* ------------------------------------------------------------ plan life */
@@ -215,10 +249,11 @@ int fmma_plan_create(fmma_plan *plan, int dim, int dtype);
 /* Replaces ~FMMA (src/fmma.cpp:26-29).  Frees device memory. */
 int fmma_plan_destroy(fmma_plan plan);
 
-/* FMMA::set_solver_type (include/fmma/fmma.hpp:92-95).  "exact", "exact_matvec"
- * and "fmm" run on the GPU; "tree" and "nrnmm" are accepted here and make
- * solve return FMMA_ERR_UNSUPPORTED; anything else makes solve return
- * FMMA_ERR_SOLVER, the reference's exit path (src/fmma.cpp:183-186). */
+/* FMMA::set_solver_type (include/fmma/fmma.hpp:92-95).  "exact",
+ * "exact_matvec", "fmm", "tree" and "nrnmm" run on the GPU (the last two need
+ * a launcher table with m2p, which every built-in and set_fn table has);
+ * anything else makes solve return FMMA_ERR_SOLVER, the reference's exit path
+ * (src/fmma.cpp:183-186). */
 int fmma_plan_set_solver_type(fmma_plan plan, const char *type);
 
 /* FMMA::set_poly_ord (include/fmma/fmma.hpp:108-111): n = poly_ord+1 nodes per
@@ -226,17 +261,20 @@ int fmma_plan_set_solver_type(fmma_plan plan, const char *type);
 int fmma_plan_set_poly_ord(fmma_plan plan, int poly_ord);
 
 /* FMMA::set_Depth (include/fmma/fmma.hpp:116-119).  <= 0 selects the auto
- * rule (int)(ln N_targets / DIM) + 1 at the next fmm solve, which then sticks
- * (src/fmm.cpp:23-28). */
+ * rule (int)(ln N_targets / DIM) + 1 at the next fmm or tree solve, which
+ * then sticks (src/fmm.cpp:23-28, src/tree.cpp:20-25). */
 int fmma_plan_set_depth(fmma_plan plan, int depth);
 int fmma_plan_get_depth(fmma_plan plan, int *depth);
 
 /* FMMA::set_trans_sym_flag (include/fmma/fmma.hpp:124-127). */
 int fmma_plan_set_trans_sym_flag(fmma_plan plan, int flag);
 
-/* FMMA::set_nrn_N (include/fmma/fmma.hpp:100-103); stored for the nrnmm
- * solver, which this library does not run. */
+/* FMMA::set_nrn_N (include/fmma/fmma.hpp:100-103): the nrnmm solver's boxes
+ * per side.  <= 0 selects the auto rule (int)(sqrt(3) M^(1/(2 DIM))) + 1, M
+ * the source count, at the next nrnmm solve, which then sticks
+ * (src/nrnmm.cpp:20-22).  The getter has no counterpart in the reference. */
 int fmma_plan_set_nrn_N(fmma_plan plan, int nrn_N);
+int fmma_plan_get_nrn_N(fmma_plan plan, int *nrn_N);
 
 /* Stands in for FMMA::set_fn / set_fn1 / set_fn2 (include/fmma/fmma.hpp:65-87)
  * with a built-in kernel.  params may be NULL when nparams == 0. */

@@ -214,6 +214,33 @@ int main()
             for (size_t i = 0; i < nt; ++i) host[j] += cot[i] * fn(t[i], s[j]);
         check("general_transpose_exact_vs_host", rel(out, host), 1e-13);
     }
+    {  // tree and nrnmm with a user lambda vs the built-in kernel, and vs the exact sum
+        const size_t n = 8000;
+        auto x = pts<double, 3>(n);
+        std::vector<double> w(n);
+        for (auto &v : w) v = uni();
+        fmma::FMMA<double, 3> f, g;
+        f.set_fn([] __host__ __device__(const std::array<double, 3> &d) {
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            return r2 == 0.0 ? 0.0 : 1.0 / std::sqrt(r2);
+        });
+        g.set_kernel(FMMA_KERNEL_INV_R_REG);
+        for (auto *p : {&f, &g}) {
+            p->set_poly_ord(4);
+            p->set_Depth(4);
+            p->set_nrn_N(6);
+        }
+        std::vector<double> ta, tb, na, nb, e;
+        f.tree(x, w, x, ta);
+        g.tree(x, w, x, tb);
+        f.nrnmm(x, w, x, na);
+        g.nrnmm(x, w, x, nb);
+        f.exact(x, w, x, e);
+        check("tree_lambda_vs_builtin", rel(ta, tb), 1e-12);
+        check("nrnmm_lambda_vs_builtin", rel(na, nb), 1e-12);
+        check("tree_vs_exact", rel(ta, e), 1e-3);
+        check("nrnmm_vs_exact", rel(na, e), 1e-3);
+    }
     {  // time log keys of the reference (src/fmm.cpp:57-141, src/fmma.cpp:189)
         fmma::FMMA<double, 2> f;
         auto x = pts<double, 2>(5000);

@@ -232,9 +232,6 @@ def test_solver_errors(gpu):
     with pytest.raises(fmma_amd.FMMAError) as e:
         gpu_solve(gpu, "nosuch", t, None, t)
     assert e.value.code == 3
-    with pytest.raises(fmma_amd.FMMAError) as e:
-        gpu_solve(gpu, "tree", t, None, t)
-    assert e.value.code == 5
 
 
 def test_repeated_solves_are_deterministic(gpu):
