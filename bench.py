@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the batching-form measurement")
     ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--no-solvers", action="store_true", help="skip the tree and nrnmm measurement")
+    ap.add_argument("--solvers-only", action="store_true", help="only the tree and nrnmm measurement")
+    ap.add_argument("--solvers-n", type=int, default=1 << 20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -207,6 +210,130 @@ def batch_bench(fmma_amd, synth, torch, nprob, steps):
             "max_abs_diff_vs_loop": diff, "m2l_tiles": {str(k): v for k, v in pl.m2l_tiles().items()}}
 
 
+def _clip_window(b, N, w):
+    """Boxes of a window [2(b//2) - 2, 2(b//2) + 3] (w = 6) or [b - 1, b + 1]
+    (w = 3) that lie in [0, N), per coordinate."""
+    if w == 6:
+        lo = 2 * (b // 2) - 2
+        return np.minimum(lo + 6, N) - np.maximum(lo, 0)
+    return np.minimum(b + 2, N) - np.maximum(b - 1, 0)
+
+
+def m2p_evaluations(x_np, P, depth=None, grid=None):
+    """Kernel evaluations of the M2P stage (targets x far-box nodes): tree --
+    every level 2..Depth-1, the interaction list of the target's box; nrnmm --
+    every box of the grid at Chebyshev distance >= 2."""
+    lo, hi = x_np.min(0), x_np.max(0)
+    Len = float(np.max(hi - lo))
+    org = (hi + lo) / 2 - Len / 2
+    total = 0
+    if grid is not None:
+        b = np.minimum(((x_np - org) / (Len / grid)).astype(np.int64), grid - 1)
+        near = np.prod(_clip_window(b, grid, 3), axis=1)
+        return int(((grid ** 3 - near) * P).sum())
+    L = depth - 1
+    bl = np.minimum(((x_np - org) / (Len / (1 << L))).astype(np.int64), (1 << L) - 1)
+    for l in range(2, L + 1):
+        b = bl >> (L - l)
+        N = 1 << l
+        cnt = np.prod(_clip_window(b, N, 6), axis=1) - np.prod(_clip_window(b, N, 3), axis=1)
+        total += int(cnt.sum()) * P
+    return total
+
+
+def solver_bench(fmma_amd, synth, torch, n, steps, with_cpu):
+    """The tree and nrnmm solvers (src/tree.cpp, src/nrnmm.cpp) on N points
+    uniform in the unit cube, targets = sources, f = 1/r (0 at r = 0), 4 nodes
+    per dimension, auto Depth / nrn_N.  GPU: best of `steps` solves with the
+    stage times; roofline of the M2P stage against FP64 peak.  CPU: the
+    compiled reference (or the C restatement) with two target samples of the
+    points (tree: 8192 and 65536, nrnmm: 2048 and 4096) and all N sources,
+    Depth / nrn_N fixed to the GPU's: the per-target cost (from the
+    difference of the two) times N plus the rest."""
+    x_np = synth.points(1, n, 3)
+    w_np = synth.uniform(2, n)
+    x = torch.from_numpy(x_np).cuda()
+    w = torch.from_numpy(w_np).cuda()
+    out = {}
+    for solver in ("tree", "nrnmm"):
+        pl = fmma_amd.Plan(3, "float64")
+        pl.set_solver_type(solver)
+        pl.set_poly_ord(3)
+        pl.set_kernel("inv_r_reg")
+        a = torch.zeros(n, dtype=torch.float64, device="cuda")
+        pl.solve(x, w, x, a)  # warm-up; fixes the auto Depth / nrn_N
+        torch.cuda.synchronize()
+        best, log = None, None
+        for _ in range(steps):
+            pl.clear_time_log()
+            t0 = time.perf_counter()
+            pl.solve(x, w, x, a)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            if best is None or ms < best:
+                best, log = ms, {k: round(v, 3) for k, v in pl.time_log().items() if not k.startswith("total")}
+        size = pl.depth if solver == "tree" else pl.nrn_N
+        ev = m2p_evaluations(x_np, 64, depth=size) if solver == "tree" else m2p_evaluations(x_np, 64, grid=size)
+        m2p_s = log["M2P"] * 1e-3
+        flops = ev * P2P_FLOPS_PER_PAIR
+        rec = {"ms": round(best, 3), "stages_ms": log, ("depth" if solver == "tree" else "nrn_N"): size,
+               "m2p_evaluations": ev,
+               "m2p_roofline": {"kernel": "m2p_kernel", "bound": "fp64-valu",
+                                "achieved": float("%.4g" % (flops / m2p_s / 1e12)), "peak": PEAK_FP64 / 1e12,
+                                "unit": "TFLOP/s", "frac": float("%.4g" % (flops / m2p_s / PEAK_FP64)),
+                                "work": "%d evaluations x %d flops" % (ev, P2P_FLOPS_PER_PAIR)}}
+        print("# %s (%s %d): %.2f ms, stages %s, M2P %.3g evaluations/s" % (
+            solver, "Depth" if solver == "tree" else "nrn_N", size, best, json.dumps(log), ev / m2p_s),
+            file=sys.stderr, flush=True)
+        if with_cpu:
+            rec["cpu_baseline"] = solver_cpu(solver, x_np, w_np, size)
+            rec["speedup_vs_cpu"] = round(rec["cpu_baseline"]["value"] / best, 1)
+            print("# %s CPU %s" % (solver, json.dumps(rec["cpu_baseline"])), file=sys.stderr, flush=True)
+        out[solver] = rec
+    out["workload"] = ("3-D 1/r (0 at r=0), N=%d uniform points in the unit cube, targets = sources, weights U[0,1), "
+                       "4 Chebyshev nodes/dim, auto Depth / nrn_N" % n)
+    return out
+
+
+def solver_cpu(solver, x_np, w_np, size):
+    from oracle import pyoracle
+    n = x_np.shape[0]
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    ref = pyoracle.have_ref()
+    if ref:
+        pyoracle.ref_set_threads(cores)
+    rng = np.random.default_rng(5)
+    times = {}
+    sizes = (8192, 65536) if solver == "tree" else (2048, 4096)
+    for s in sizes:
+        t = np.ascontiguousarray(x_np[rng.choice(n, s, replace=False)])
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            if ref:
+                if solver == "tree":
+                    pyoracle.ref_solve("tree", t, w_np, x_np, 3, size, 0, 1)
+                else:
+                    pyoracle.ref_solve("nrnmm", t, w_np, x_np, 3, -1, 0, 1, nrn_N=size)
+            elif solver == "tree":
+                pyoracle.tree(t, w_np, x_np, 3, size, 1)
+            else:
+                pyoracle.nrnmm(t, w_np, x_np, 3, size, 1)
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        times[s] = best
+    s0, s1 = sizes
+    per_target = max(times[s1] - times[s0], 0.0) / (s1 - s0)
+    rest = max(times[s0] - per_target * s0, 0.0)
+    ms = (rest + per_target * n) * 1e3
+    return {"value": round(ms, 1), "unit": "ms (extrapolated to N=%d targets)" % n, "cores": cores,
+            "kind": "reference" if ref else "port",
+            "sample": "%s solve with %d and %d of the points as targets, all %d as sources: %.3f s and %.3f s; "
+                      "per-target cost x N + the rest" % (solver, s0, s1, n, times[s0], times[s1])}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -228,6 +355,10 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batching": batch_bench(fmma_amd, synth, torch, args.batch_problems, args.steps)}))
+        return
+    if args.solvers_only:
+        print(json.dumps({"solvers": solver_bench(fmma_amd, synth, torch, args.solvers_n, max(3, args.steps // 2),
+                                                  not args.no_cpu)}))
         return
     n = args.n
     x_np = synth.points(1, n, 3)
@@ -331,6 +462,10 @@ def main():
     if not args.no_batch:
         batching = batch_bench(fmma_amd, synth, torch, args.batch_problems, max(3, args.steps // 2))
 
+    solvers = None
+    if not args.no_solvers:
+        solvers = solver_bench(fmma_amd, synth, torch, args.solvers_n, 3, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -359,6 +494,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batching": batching,
+        "solvers": solvers,
     }
     if cpu and "native" in results:
         line["speedup_fp64_gpu_vs_cpu"] = round(cpu["value"] / results["native"]["ms"], 1)
