@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--depth", type=int, default=-1)
+    ap.add_argument("--poly-ord", type=int, default=3, help="Chebyshev order (nodes per dimension - 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--forms", default="fp32,native,bf16x3")
@@ -372,7 +373,7 @@ def main():
     for form in forms:
         pl = fmma_amd.Plan(3, "float64")
         pl.set_solver_type("fmm")
-        pl.set_poly_ord(3)
+        pl.set_poly_ord(args.poly_ord)
         pl.set_Depth(args.depth)
         pl.set_trans_sym_flag(1)
         pl.set_kernel("inv_r_reg")
@@ -432,7 +433,7 @@ def main():
     evaluated = self_pairs + (pairs - self_pairs) // 2
     mpairs = m2l_pairs(depth)
     dom = max(stages, key=stages.get)
-    P = 64
+    P = (args.poly_ord + 1) ** 3
     if dom == "P2P":
         flops = pairs * P2P_FLOPS_PER_PAIR
         ach = flops / (stages["P2P"] * 1e-3)
@@ -484,8 +485,9 @@ def main():
         "dtype": "f64 (M2L %s)" % head,
         "data": "synthetic",
         "config": {"workload": "3-D 1/r FMM solve, N=%d uniform points in the unit cube, targets=sources, "
-                               "4 Chebyshev nodes/dim, Depth %d, trans_sym 1, M2L form %s" % (n, depth, head),
-                   "n": n, "poly_ord": 3, "depth": depth, "kernel": "1/|x-y| (0 at r=0)", "m2l": head,
+                               "%d Chebyshev nodes/dim, Depth %d, trans_sym 1, M2L form %s"
+                               % (n, args.poly_ord + 1, depth, head),
+                   "n": n, "poly_ord": args.poly_ord, "depth": depth, "kernel": "1/|x-y| (0 at r=0)", "m2l": head,
                    "parallelism": "single GPU"},
         "forms": results,
         "speedup_fp32_vs_fp64_gpu": round(results["native"]["ms"] / ms_head, 3) if "native" in results else None,

@@ -261,6 +261,11 @@ class Plan:
         the first batched solve of a layout (default on)."""
         _lib.check(self._L.fmma_plan_set_m2l_autotune(self._h, 1 if on else 0))
 
+    def set_m2l_tile(self, tile):
+        """Force the M2L tile width (target boxes per workgroup: 16, 32, 64,
+        128, 256) wherever a pipelined kernel has it; 0 restores the tuner."""
+        _lib.check(self._L.fmma_plan_set_m2l_tile(self._h, int(tile)))
+
     def m2l_tiles(self, batch=True):
         """{parent boxes per level: M2L tile width} chosen by the autotuner
         for batches (batch=True) or single solves."""

@@ -117,7 +117,12 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
 void m2l_shape(int P, int mode, int &MP, int &KP)
 {
     MP = (P + 15) / 16 * 16;
-    KP = mode == 2 ? (P + 31) / 32 * 32 : (P + 3) / 4 * 4;
+    // beyond 128 the pipelined contraction takes chunks of 32 (FP64, FP32)
+    // or 128 (BF16x3) of the padded length
+    if (mode == 2)
+        KP = P <= 128 ? (P + 31) / 32 * 32 : (P + 127) / 128 * 128;
+    else
+        KP = P <= 128 ? (P + 3) / 4 * 4 : (P + 31) / 32 * 32;
 }
 
 size_t m2l_frag_elems(int n_far, int P, int mode)
@@ -383,28 +388,34 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
     }
 }
 
-// Software-pipelined contraction, compile-time padded length KP and tile
-// width TB (target boxes per workgroup, 64..256).  Thread t gathers TPC = 256/TB
-// threads' share of one target's partner: column t / TPC, elements
-// [(t % TPC) * EPT, +EPT) of the padded operand copy, with 16-byte loads, for
-// offset i+1 while the MFMAs of offset i read LDS; the operator fragments of
-// offset i+1 are prefetched the same way.  Accumulators stay in registers
-// across all offsets: FP64 in MODE 0, FP32 (the MFMA accumulator) in MODES 1-2.
-template <typename TW, int D, int MODE, int KP, int TB>
+// Software-pipelined contraction, compile-time padded length KP, tile width
+// TB (target boxes per workgroup, 16..256) and contraction chunk KC (KP
+// itself up to 128; for longer rows the contraction is cut into KP/KC steps
+// per offset so that operand tiles and operator fragments stay small).  A
+// step is (offset, chunk).  Thread t gathers TPC = 256/TB threads' share of
+// one target's partner chunk: column t / TPC, elements [(t % TPC) * EPT,
+// +EPT) of the chunk of the padded operand copy, with 16-byte loads, for step
+// j+1 while the MFMAs of step j read LDS; the operator fragments of step j+1
+// are prefetched the same way.  Accumulators stay in registers across all
+// steps: FP64 in MODE 0, FP32 (the MFMA accumulator) in MODES 1-2.
+template <typename TW, int D, int MODE, int KP, int TB, int KC>
 __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 {
-    constexpr int NKS = MODE == 2 ? KP / 32 : KP / 4;
-    constexpr int TPC = 256 / TB, EPT = KP / TPC, NCB = TB / 16;
+    static_assert(KP % KC == 0, "the chunk divides the padded length");
+    constexpr int NCH = KP / KC;                               // chunks per offset
+    constexpr int NKS = MODE == 2 ? KP / 32 : KP / 4;          // k-steps per offset
+    constexpr int NKC = MODE == 2 ? KC / 32 : KC / 4;          // k-steps per chunk
+    constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 16;
     using CT = typename std::conditional<MODE == 0, double, float>::type;
     using V4 = typename std::conditional<MODE == 0, d4_t, f4_t>::type;
     using ST = typename std::conditional<MODE == 2, __bf16, CT>::type;
     using AT = typename std::conditional<MODE == 2, bf8_t, CT>::type;
-    constexpr int LDB = MODE == 2 ? KP + 8 : KP + 1;
+    constexpr int LDB = MODE == 2 ? KC + 8 : KC + 1;
     constexpr int NPART = MODE == 2 ? 2 : 1;
     constexpr int VEC = 16 / (int)sizeof(ST);  // elements per 16-byte load
     static_assert(EPT % VEC == 0, "gather chunks must be whole 16-byte vectors");
-    // Two tiles when they fit: offset i+1's tile is written while offset i's
-    // MFMAs run, one barrier per offset instead of two.
+    // Two tiles when they fit: step j+1's tile is written while step j's
+    // MFMAs run, one barrier per step instead of two.
     constexpr int BUF = NPART * TB * LDB;
     constexpr bool DB = 2 * BUF * (int)sizeof(ST) <= kM2LDoubleLds;
     __shared__ __align__(16) ST Bs[(DB ? 2 : 1) * BUF];
@@ -436,7 +447,8 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 
     uint4 g[EPT / VEC];
     [[maybe_unused]] uint4 g_lo[MODE == 2 ? EPT / VEC : 1];
-    auto gather = [&](int i) {
+    auto gather = [&](int j) {
+        const int i = NCH == 1 ? j : j / NCH, c = NCH == 1 ? 0 : j - i * NCH;
         int m[D];
         bool ok = cvalid;
 #pragma unroll
@@ -444,35 +456,36 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
             m[d] = tc[d] + cls_o[i * D + d];
             ok = ok && m[d] >= 0 && m[d] < N;
         }
-        const size_t off = (size_t)(ok ? morton_encode<D>(m) : 0u) * KP + part * EPT;
+        const size_t off = (size_t)(ok ? morton_encode<D>(m) : 0u) * KP + c * KC + part * EPT;
         const uint4 *src = reinterpret_cast<const uint4 *>(Wg + off);
 #pragma unroll
-        for (int j = 0; j < EPT / VEC; ++j) g[j] = ok ? src[j] : uint4{0, 0, 0, 0};
+        for (int e = 0; e < EPT / VEC; ++e) g[e] = ok ? src[e] : uint4{0, 0, 0, 0};
         if constexpr (MODE == 2) {
             const uint4 *sl = reinterpret_cast<const uint4 *>(Wg_lo + off);
 #pragma unroll
-            for (int j = 0; j < EPT / VEC; ++j) g_lo[j] = ok ? sl[j] : uint4{0, 0, 0, 0};
+            for (int e = 0; e < EPT / VEC; ++e) g_lo[e] = ok ? sl[e] : uint4{0, 0, 0, 0};
         }
     };
-    AT afr[NKS];
-    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKS : 1];
-    auto load_a = [&](int i) {
+    AT afr[NKC];
+    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKC : 1];
+    auto load_a = [&](int j) {
         if (!active) return;
+        const int i = NCH == 1 ? j : j / NCH, c = NCH == 1 ? 0 : j - i * NCH;
         const size_t f = (size_t)cls_f[i];
-        const size_t base = (f * nrb + rb) * NKS * 64;
+        const size_t base = ((f * nrb + rb) * NKS + (size_t)c * NKC) * 64;
         if constexpr (MODE == 2) {
             const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
             const bf8_t *Kl =
                 reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) {
+            for (int s = 0; s < NKC; ++s) {
                 afr[s] = Kh[base + s * 64 + lane];
                 afr_lo[s] = Kl[base + s * 64 + lane];
             }
         } else {
             const CT *Kf = static_cast<const CT *>(a.Kf) + task.k_off;
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) afr[s] = Kf[base + s * 64 + lane];
+            for (int s = 0; s < NKC; ++s) afr[s] = Kf[base + s * 64 + lane];
         }
     };
 
@@ -480,19 +493,19 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = V4{0, 0, 0, 0};
 
-    const int n = a.n_cls;
+    const int n = a.n_cls * NCH;
     auto store_tile = [&](ST *tile) {  // registers -> LDS tile
         ST *dst = tile + col * LDB + part * EPT;
         if constexpr (MODE == 2) {
 #pragma unroll
-            for (int j = 0; j < EPT / VEC; ++j) {
-                reinterpret_cast<uint4 *>(dst)[j] = g[j];
-                reinterpret_cast<uint4 *>(dst + TB * LDB)[j] = g_lo[j];
+            for (int e = 0; e < EPT / VEC; ++e) {
+                reinterpret_cast<uint4 *>(dst)[e] = g[e];
+                reinterpret_cast<uint4 *>(dst + TB * LDB)[e] = g_lo[e];
             }
         } else {
             const ST *v = reinterpret_cast<const ST *>(g);
 #pragma unroll
-            for (int j = 0; j < EPT; ++j) dst[j] = v[j];
+            for (int e = 0; e < EPT; ++e) dst[e] = v[e];
         }
     };
     gather(0);
@@ -507,10 +520,10 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
             store_tile(Bs);
             __syncthreads();
         }
-        AT acur[NKS];
-        [[maybe_unused]] AT acur_lo[MODE == 2 ? NKS : 1];
+        AT acur[NKC];
+        [[maybe_unused]] AT acur_lo[MODE == 2 ? NKC : 1];
 #pragma unroll
-        for (int s = 0; s < NKS; ++s) {
+        for (int s = 0; s < NKC; ++s) {
             acur[s] = afr[s];
             if constexpr (MODE == 2) acur_lo[s] = afr_lo[s];
         }
@@ -521,7 +534,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         if (active) {
             if constexpr (MODE == 2) {
 #pragma unroll
-                for (int s = 0; s < NKS; ++s) {
+                for (int s = 0; s < NKC; ++s) {
                     const int kk = 32 * s + 8 * (lane >> 4);
 #pragma unroll
                     for (int cb = 0; cb < NCB; ++cb) {
@@ -535,7 +548,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                 }
             } else {
 #pragma unroll
-                for (int s = 0; s < NKS; ++s) {
+                for (int s = 0; s < NKC; ++s) {
                     const int kk = 4 * s + (lane >> 4);
 #pragma unroll
                     for (int cb = 0; cb < NCB; ++cb) {
@@ -568,16 +581,22 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     }
 }
 
+// The contraction chunk of a padded length: all of it up to 128; beyond,
+// 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a multiple of
+// 128), so that a step's MFMA work still covers the next step's loads.
+constexpr int m2l_chunk(int KP, int mode) { return KP <= 128 ? KP : mode == 2 ? 128 : 32; }
+
 // Tile widths with a pipelined kernel for (MODE, KP, TB); the others use
 // m2l_mfma4 / m2l_bf16x3.
 template <typename T, int D, int MODE, int KP, int TB>
 static bool launch_pipe_tb(const M2LKArgs &a, dim3 grid, hipStream_t st)
 {
     constexpr int VEC = MODE == 2 ? 8 : MODE == 1 ? 4 : 2;
-    if constexpr ((KP / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128)) {
+    constexpr int KC = m2l_chunk(KP, MODE);
+    if constexpr ((KC / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128)) {
         return false;
     } else {
-        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC>), grid, dim3(256), 0, st, a);
         return true;
     }
 }
@@ -587,8 +606,10 @@ static bool launch_pipe_kp(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 {
     if (tb == 64) return launch_pipe_tb<T, D, MODE, KP, 64>(a, grid, st);
     if constexpr (D == 3) {
-        if (tb == 16) return launch_pipe_tb<T, D, MODE, KP, 16>(a, grid, st);
-        if (tb == 32) return launch_pipe_tb<T, D, MODE, KP, 32>(a, grid, st);
+        if constexpr (KP <= 128) {
+            if (tb == 16) return launch_pipe_tb<T, D, MODE, KP, 16>(a, grid, st);
+            if (tb == 32) return launch_pipe_tb<T, D, MODE, KP, 32>(a, grid, st);
+        }
         if (tb == 128) return launch_pipe_tb<T, D, MODE, KP, 128>(a, grid, st);
         if (tb == 256) return launch_pipe_tb<T, D, MODE, KP, 256>(a, grid, st);
     }
@@ -604,17 +625,37 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
     case 64: return launch_pipe_kp<T, D, MODE, 64>(a, tb, grid, st);
     case 128: return launch_pipe_kp<T, D, MODE, 128>(a, tb, grid, st);
     }
+    if constexpr (D == 3) {
+        // 6, 7 and 8 nodes per dimension
+        if constexpr (MODE == 2) {
+            switch (a.KP) {
+            case 256: return launch_pipe_kp<T, D, MODE, 256>(a, tb, grid, st);
+            case 384: return launch_pipe_kp<T, D, MODE, 384>(a, tb, grid, st);
+            case 512: return launch_pipe_kp<T, D, MODE, 512>(a, tb, grid, st);
+            }
+        } else {
+            switch (a.KP) {
+            case 224: return launch_pipe_kp<T, D, MODE, 224>(a, tb, grid, st);
+            case 352: return launch_pipe_kp<T, D, MODE, 352>(a, tb, grid, st);
+            case 512: return launch_pipe_kp<T, D, MODE, 512>(a, tb, grid, st);
+            }
+        }
+    }
     return false;
 }
 
 bool m2l_pipe_supported(int D, int KP, int mode, int tb)
 {
     const int vec = mode == 2 ? 8 : mode == 1 ? 4 : 2;
-    const bool kp_ok = KP == 32 || KP == 36 || KP == 64 || KP == 128;
-    if (!kp_ok || (mode == 2 && KP % 32) || (KP / (256 / tb)) % vec) return false;
+    const bool small = KP == 32 || KP == 36 || KP == 64 || KP == 128;
+    const bool big = D == 3 && (mode == 2 ? (KP == 256 || KP == 384 || KP == 512)
+                                          : (KP == 224 || KP == 352 || KP == 512));
+    if (!small && !big) return false;
+    const int kc = m2l_chunk(KP, mode);
+    if ((mode == 2 && KP % 32) || (kc / (256 / tb)) % vec) return false;
     if (tb == 64) return true;
     if (D != 3) return false;
-    if (tb == 16 || tb == 32) return true;
+    if (tb == 16 || tb == 32) return small;
     return mode == 0 ? tb == 128 : (tb == 128 || tb == 256);
 }
 

@@ -144,6 +144,7 @@ struct Plan {
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
     std::unique_ptr<BatchPlan> batch;
     bool batch_graph = true, m2l_autotune = true;
+    int forced_tb = 0;                    // > 0: this M2L tile width wherever a pipelined kernel has it
     std::map<long, int> tuned_tb;        // batches: (dim, P, mode, parents per level) -> tile width
     std::map<long, int> tuned_tb_single; // single problems: (dim, P, mode, parents per level) -> tile width
     int tuned_set_L = -1, tuned_set_mode = -1;  // the single-problem tasks carry tuned widths
@@ -381,6 +382,10 @@ static int tune_single(Plan &pl, const M2LKArgs &ka, int L)
     int tb_of[kMaxLevels] = {};
     for (int l = 2; l <= L; ++l) {
         const long npar = 1L << (D * (l - 1));
+        if (pl.forced_tb > 0) {
+            tb_of[l] = m2l_pipe_supported(D, ka.KP, mode, pl.forced_tb) ? pl.forced_tb : 0;
+            continue;
+        }
         const long key = tune_key(D, P, mode, npar);
         auto it = pl.tuned_tb_single.find(key);
         if (it != pl.tuned_tb_single.end()) {
@@ -546,7 +551,8 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
                 const long nbox = (pl.lvl_off[L + 1] - pl.lvl_off[2]) / P;
                 if ((rc = m2l_convert<T>(Wm + pl.lvl_off[2], nbox, P, ka.KP, mode, wc, wl, st))) return rc;
             }
-            if (pl.m2l_autotune && pl.tuned_set_L != L && (rc = tune_single<T, D>(pl, ka, L))) return rc;
+            if ((pl.m2l_autotune || pl.forced_tb > 0) && pl.tuned_set_L != L && (rc = tune_single<T, D>(pl, ka, L)))
+                return rc;
             ka.tasks = pl.d_tasks.as<M2LTask>();
             if ((rc = m2l_apply<T, D>(ka, pl.groups, mode, st))) return rc;
         } else {
@@ -1348,13 +1354,14 @@ static int batch_typed(Plan &pl, int nprob, const long *tc, const void *tg_, con
     BatchPlan &bp = *pl.batch;
     if ((rc = batch_buffers<T, D>(pl, bp))) return rc;
     if (!bp.tuned) {
-        if (!pl.m2l_autotune) {
+        if (!pl.m2l_autotune || pl.forced_tb > 0) {
             std::map<long, int> choice;
             int MP, KP;
             m2l_shape(pl.P(), pl.mode(), MP, KP);
             for (int l = 2; l <= bp.Lmax; ++l) {
                 const long npar = 1L << (D * (l - 1));
-                choice[npar] = rule_tb(D, KP, pl.mode(), npar);
+                const bool forced = pl.forced_tb > 0 && m2l_pipe_supported(D, KP, pl.mode(), pl.forced_tb);
+                choice[npar] = forced ? pl.forced_tb : rule_tb(D, KP, pl.mode(), npar);
             }
             std::vector<M2LTask> tasks = batch_tasks(pl, bp, D, bp.groups, choice);
             if ((rc = upload(bp.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), pl.stream))) return rc;
@@ -1612,6 +1619,22 @@ int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.m2l_autotune = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_m2l_tile(fmma_plan plan, int tile)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    if (tile != 0 && tile != 16 && tile != 32 && tile != 64 && tile != 128 && tile != 256)
+        return fmma::lib::fail(FMMA_ERR_INVALID, "M2L tile width must be 0, 16, 32, 64, 128 or 256");
+    fmma::lib::Plan &pl = plan->pl;
+    pl.forced_tb = tile;
+    pl.set_L = -1;  // rebuild the single-solve tasks
+    pl.tuned_set_L = -1;
+    if (pl.batch) {
+        pl.batch->tuned = false;
+        pl.batch->drop_graph();
+    }
     return FMMA_OK;
 }
 

@@ -362,6 +362,11 @@ int fmma_plan_solve_batch(fmma_plan plan, int n_problems, const long *t_counts, 
                           const void *weights, const long *s_counts, const void *sources, void *ans);
 int fmma_plan_set_batch_graph(fmma_plan plan, int enable);   /* default 1 */
 int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable);  /* default 1 */
+/* 0 (default): the autotuner's (or, with it off, the rule's) M2L tile widths;
+ * 16, 32, 64, 128 or 256: that many target boxes per workgroup at every level
+ * where a pipelined kernel has the width (the rule's elsewhere), for tests and
+ * measurements.  No reference counterpart. */
+int fmma_plan_set_m2l_tile(fmma_plan plan, int tile);
 /* The M2L tile autotuner's choices so far, for batches (batch = 1) or single
  * solves (0): tile width (target boxes per workgroup) per number of parent
  * boxes of a level.  Returns the number of entries.  The single-solve tuner

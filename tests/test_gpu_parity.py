@@ -253,3 +253,27 @@ def test_m2l_forms_at_512_coefficients(gpu, oracle, prec, tol):
     err = rel_l2(got, ref)
     print("  P=512 %s vs oracle %.3e" % (prec, err))
     assert err <= tol, err
+
+
+_TILE_REF = {}
+
+
+@pytest.mark.parametrize("p", [3, 5, 6])
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-6)])
+@pytest.mark.parametrize("tile", [16, 32, 64, 128, 256])
+def test_m2l_every_tile_width(gpu, oracle, p, prec, tol, tile):
+    """Every pipelined M2L kernel (tile widths 16..256 target boxes, the
+    chunked contraction for 6 and 7 nodes per dimension) against the oracle,
+    Depth 4 (full and partial tiles)."""
+    x = synth.points(95, 4000, 3)
+    w = synth.uniform(96, 4000)
+    if p not in _TILE_REF:
+        _TILE_REF[p] = oracle.fmm(x, w, x, p, 4, 1, KID["inv_r_reg"])
+    pl = gpu.Plan(3, np.float64)
+    pl.set_Depth(4)
+    pl.set_m2l_tile(tile)
+    got, _ = gpu_solve(gpu, "fmm", x, w, x, p, 4, 1, "inv_r_reg", shared=True, precision=prec, plan=pl)
+    assert pl.depth == 4
+    err = rel_l2(got, _TILE_REF[p])
+    print("  p=%d %s tile %d: %.3e, tiles %s" % (p, prec, tile, err, pl.m2l_tiles(batch=False)))
+    assert err <= tol
