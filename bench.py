@@ -16,6 +16,11 @@ FP64 GPU fmm over all N targets; total error against the FP64 exact sum on
 the dominant kernel (live HIP-event stage times), and the CPU baseline: the
 compiled reference (oracle/_ref) or its C restatement (oracle/) timed on this
 host on a bounded sample and scaled to N.
+
+Beside the headline: the same workload at the Chebyshev orders whose error
+against the exact sum is <= 1e-6 (5 and 6 nodes per dimension, Depth 6) in
+every form (`at_rel_err_1e-6`), the batching form (512 Yukawa problems), and
+the tree and nrnmm solvers at N = 1,048,576.
 """
 import argparse
 import json
@@ -52,6 +57,9 @@ def parse():
     ap.add_argument("--batch-only", action="store_true", help="only the batching-form measurement")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--no-solvers", action="store_true", help="skip the tree and nrnmm measurement")
+    ap.add_argument("--accuracy-configs", default="4:6,5:6",
+                    help="poly_ord:Depth pairs also timed in every form with their errors vs the exact sum "
+                         "(the orders that reach rel-err <= 1e-6); empty to skip")
     ap.add_argument("--solvers-only", action="store_true", help="only the tree and nrnmm measurement")
     ap.add_argument("--solvers-n", type=int, default=1 << 20)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -335,6 +343,55 @@ def solver_cpu(solver, x_np, w_np, size):
                       "per-target cost x N + the rest" % (solver, s0, s1, n, times[s0], times[s1])}
 
 
+def run_forms(fmma_amd, torch, x, w, poly_ord, depth_in, forms, steps, warmup, exact):
+    """The 3-D 1/r fmm solve (trans_sym 1, 0 at r = 0) in each M2L form: ms per
+    solve over `steps` after `warmup` (the first runs the tile autotuner),
+    HIP-event stage times, the precision error of each form against the FP64
+    form over all targets and, with exact = (indices, sums), the total error
+    against the exact sum on those targets."""
+    n = x.shape[0]
+    plans, answers, results = {}, {}, {}
+    for form in forms:
+        pl = fmma_amd.Plan(3, "float64")
+        pl.set_solver_type("fmm")
+        pl.set_poly_ord(poly_ord)
+        pl.set_Depth(depth_in)
+        pl.set_trans_sym_flag(1)
+        pl.set_kernel("inv_r_reg")
+        pl.set_m2l_precision(form)
+        plans[form] = pl
+        answers[form] = torch.zeros(n, dtype=torch.float64, device=x.device)
+    for form in forms:
+        pl, a = plans[form], answers[form]
+        for _ in range(warmup):
+            pl.solve(x, w, x, a)
+        torch.cuda.synchronize()
+        pl.clear_time_log()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pl.solve(x, w, x, a)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        log = {k: v / steps for k, v in pl.time_log().items() if not k.startswith("total")}
+        results[form] = {"ms": round(ms, 3), "stages_ms": {k: round(v, 4) for k, v in log.items()},
+                         "m2l_tiles": {str(k): v for k, v in pl.m2l_tiles(batch=False).items()}}
+        print("# p=%d depth %d form %-7s %.3f ms/solve  stages %s" % (
+            poly_ord, pl.depth, form, ms, json.dumps(results[form]["stages_ms"])), file=sys.stderr, flush=True)
+    depth = plans[forms[0]].depth
+    ref_form = "native" if "native" in forms else forms[0]
+    a_ref = answers[ref_form].cpu().numpy()
+    for form in forms:
+        if form != ref_form:
+            results[form]["precision_err_vs_fp64_fmm"] = rel_l2(answers[form].cpu().numpy(), a_ref)
+    if exact is not None:
+        idx, u = exact
+        for form in forms:
+            got = answers[form].cpu().numpy()[idx]
+            results[form]["total_err_vs_exact"] = rel_l2(got, u)
+            results[form]["rms_rel_err_vs_exact"] = rms_rel(got, u)
+    return results, depth
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -369,48 +426,9 @@ def main():
     torch.cuda.synchronize()
 
     forms = [f for f in args.forms.split(",") if f]
-    plans, answers, results = {}, {}, {}
-    for form in forms:
-        pl = fmma_amd.Plan(3, "float64")
-        pl.set_solver_type("fmm")
-        pl.set_poly_ord(args.poly_ord)
-        pl.set_Depth(args.depth)
-        pl.set_trans_sym_flag(1)
-        pl.set_kernel("inv_r_reg")
-        pl.set_m2l_precision(form)
-        plans[form] = pl
-        answers[form] = torch.zeros(n, dtype=torch.float64, device=dev)
-
-    def run(form, steps, warmup):
-        pl, a = plans[form], answers[form]
-        for _ in range(warmup):
-            pl.solve(x, w, x, a)
-        torch.cuda.synchronize()
-        pl.clear_time_log()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            pl.solve(x, w, x, a)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        log = {k: v / steps for k, v in pl.time_log().items() if not k.startswith("total")}
-        return el * 1e3 / steps, log
-
-    for form in forms:
-        ms, log = run(form, args.steps, args.warmup)
-        results[form] = {"ms": round(ms, 3), "stages_ms": {k: round(v, 4) for k, v in log.items()},
-                         "m2l_tiles": {str(k): v for k, v in plans[form].m2l_tiles(batch=False).items()}}
-        print("# form %-7s %.3f ms/solve  stages %s" % (form, ms, json.dumps(results[form]["stages_ms"])),
-              file=sys.stderr, flush=True)
-
-    depth = plans[forms[0]].depth
-    ref_form = "native" if "native" in forms else forms[0]
-    a_ref = answers[ref_form].cpu().numpy()
-    for form in forms:
-        if form != ref_form:
-            results[form]["precision_err_vs_fp64_fmm"] = rel_l2(answers[form].cpu().numpy(), a_ref)
-
-    # total error vs the FP64 exact sum on 8192 sampled targets (GPU exact solver)
+    exact = None
     if not args.no_exact:
+        # the FP64 exact sum on 8192 sampled targets (GPU exact solver), for the total error
         idx = synth.sample_indices(3, n, 8192)
         ts = x[torch.from_numpy(idx).to(dev)].contiguous()
         ex = fmma_amd.Plan(3, "float64")
@@ -418,11 +436,23 @@ def main():
         ex.set_kernel("inv_r_reg")
         u = torch.zeros(len(idx), dtype=torch.float64, device=dev)
         ex.solve(ts, w, x, u)
-        u = u.cpu().numpy()
-        for form in forms:
-            got = answers[form].cpu().numpy()[idx]
-            results[form]["total_err_vs_exact"] = rel_l2(got, u)
-            results[form]["rms_rel_err_vs_exact"] = rms_rel(got, u)
+        exact = (idx, u.cpu().numpy())
+    results, depth = run_forms(fmma_amd, torch, x, w, args.poly_ord, args.depth, forms, args.steps, args.warmup,
+                               exact)
+
+    accuracy = None
+    if exact is not None and args.accuracy_configs:
+        accuracy = {}
+        for cfg in args.accuracy_configs.split(","):
+            p, d = (int(v) for v in cfg.split(":"))
+            res, _ = run_forms(fmma_amd, torch, x, w, p, d, forms, 5, 2, exact)
+            entry = {"nodes_per_dim": p + 1, "depth": d, "forms": res}
+            if "native" in res:
+                for f in res:
+                    if f != "native":
+                        entry["speedup_%s_vs_fp64_gpu" % f] = round(res["native"]["ms"] / res[f]["ms"], 3)
+            entry["all_forms_within_1e-6"] = all(r["total_err_vs_exact"] <= 1e-6 for r in res.values())
+            accuracy["p%d_depth%d" % (p, d)] = entry
 
     head = "fp32" if "fp32" in forms else forms[0]
     ms_head = results[head]["ms"]
@@ -497,6 +527,7 @@ def main():
         "cpu_baseline": cpu,
         "batching": batching,
         "solvers": solvers,
+        "at_rel_err_1e-6": accuracy,
     }
     if cpu and "native" in results:
         line["speedup_fp64_gpu_vs_cpu"] = round(cpu["value"] / results["native"]["ms"], 1)

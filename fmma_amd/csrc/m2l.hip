@@ -397,7 +397,9 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
 // +EPT) of the chunk of the padded operand copy, with 16-byte loads, for step
 // j+1 while the MFMAs of step j read LDS; the operator fragments of step j+1
 // are prefetched the same way.  Accumulators stay in registers across all
-// steps: FP64 in MODE 0, FP32 (the MFMA accumulator) in MODES 1-2.
+// steps: FP64 in MODE 0; in MODES 1-2 the MFMA's FP32 accumulator sums one
+// offset and is then added into FP64 registers, so rounding in FP32 covers
+// KP products, not the whole interaction list.
 template <typename TW, int D, int MODE, int KP, int TB, int KC>
 __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 {
@@ -492,6 +494,13 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     V4 acc[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = V4{0, 0, 0, 0};
+    [[maybe_unused]] double accd[MODE == 0 ? 1 : NCB][4];
+    if constexpr (MODE != 0) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) accd[cb][q] = 0.0;
+    }
 
     const int n = a.n_cls * NCH;
     auto store_tile = [&](ST *tile) {  // registers -> LDS tile
@@ -560,6 +569,16 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                     }
                 }
             }
+            if constexpr (MODE != 0) {
+                if (NCH == 1 || (i + 1) % NCH == 0) {  // one offset summed: into FP64
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) accd[cb][q] += (double)acc[cb][q];
+                        acc[cb] = V4{0, 0, 0, 0};
+                    }
+                }
+            }
         }
         if constexpr (DB) {
             if (i + 1 < n) store_tile(Bs + ((i + 1) & 1) * BUF);
@@ -576,7 +595,11 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
-            if (row < P) Wl[box * P + row] = (TW)acc[cb][q];
+            if constexpr (MODE == 0) {
+                if (row < P) Wl[box * P + row] = (TW)acc[cb][q];
+            } else {
+                if (row < P) Wl[box * P + row] = (TW)accd[cb][q];
+            }
         }
     }
 }
@@ -593,7 +616,8 @@ static bool launch_pipe_tb(const M2LKArgs &a, dim3 grid, hipStream_t st)
 {
     constexpr int VEC = MODE == 2 ? 8 : MODE == 1 ? 4 : 2;
     constexpr int KC = m2l_chunk(KP, MODE);
-    if constexpr ((KC / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128)) {
+    if constexpr ((KC / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128) ||
+                  (MODE == 2 && KP > 128 && TB > 128)) {
         return false;
     } else {
         hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC>), grid, dim3(256), 0, st, a);
@@ -656,7 +680,7 @@ bool m2l_pipe_supported(int D, int KP, int mode, int tb)
     if (tb == 64) return true;
     if (D != 3) return false;
     if (tb == 16 || tb == 32) return small;
-    return mode == 0 ? tb == 128 : (tb == 128 || tb == 256);
+    return mode == 0 || (mode == 2 && !small) ? tb == 128 : (tb == 128 || tb == 256);
 }
 
 template <typename T, int D>
