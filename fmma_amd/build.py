@@ -34,9 +34,23 @@ HIP_FLAGS = [
 ]
 
 
-def _compile(entry):
+def _newest_header():
+    newest = 0.0
+    for d in (CSRC, os.path.join(ROOT, "include")):
+        for dp, _, files in os.walk(d):
+            for f in files:
+                if f.endswith((".hpp", ".h")):
+                    newest = max(newest, os.path.getmtime(os.path.join(dp, f)))
+    return newest
+
+
+def _compile(entry, incremental=False, newest_header=0.0):
     src, name, extra = entry
     obj = os.path.join(OBJ, name + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > newest_header:
+            return obj
     cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -75,11 +89,14 @@ def build_cpp_tests():
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compile every source (incremental=True: only objects older than their
+    source or any header) and link the library, pyfmma and the C++ test."""
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(SOURCES), max(1, min(16, os.cpu_count() or 4)))
+    newest = _newest_header()
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda e: _compile(e, incremental, newest), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -93,5 +110,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -18,6 +18,7 @@ __global__ __launch_bounds__(256) void batch_bbox_kernel(const T *t, const long 
                                                          const int *levels, Geom<T> *geoms)
 {
     const int p = blockIdx.x;
+    bool bad = false;
     T lo[D], hi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -34,8 +35,13 @@ __global__ __launch_bounds__(256) void batch_bbox_kernel(const T *t, const long 
                 const T v = x[i * D + d];
                 lo[d] = v < lo[d] ? v : lo[d];
                 hi[d] = hi[d] < v ? v : hi[d];
+                bad = bad || !__builtin_isfinite(v);
             }
     }
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    if (bad) s_bad = 1;
 #pragma unroll
     for (int d = 0; d < D; ++d)
         for (int o = 32; o > 0; o >>= 1) {
@@ -72,6 +78,7 @@ __global__ __launch_bounds__(256) void batch_bbox_kernel(const T *t, const long 
         for (int d = D; d < 3; ++d) g->origin[d] = T(0);
         g->Len = Len;
         g->len_leaf = Len / (T)(1u << levels[p]);
+        g->status = s_bad ? kGeomNonFinite : Len == T(0) ? kGeomZeroLen : kGeomOk;
     }
 }
 

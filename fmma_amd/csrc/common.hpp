@@ -39,7 +39,15 @@ struct Geom {
     T origin[3];
     T Len;
     T len_leaf;  // Len / (TYPE)N_L, N_L = 1 << leaf_level (src/fmm.cpp:58,129)
+    int status;  // kGeomOk, or why the cube cannot be binned
 };
+
+// Geom::status.  The reference divides by Len without a check
+// (src/fmm_core.cpp:261): with every point at one position, or a non-finite
+// coordinate, its box indices are undefined.  Here the solve is rejected.
+constexpr int kGeomOk = 0;
+constexpr int kGeomNonFinite = 1;  // a coordinate is NaN or infinite
+constexpr int kGeomZeroLen = 2;    // Len == 0: every point at one position
 
 // Leaf coordinate along one dimension: min((int)((x-o)/len), N-1) with IEEE
 // subtraction and division (src/fmm_core.cpp:261, 595; src/fmm.cpp:130).
@@ -48,8 +56,9 @@ __device__ __forceinline__ int leaf_coord(T x, T o, T len, int N)
 {
 #pragma clang fp contract(off)
     const T q = (x - o) / len;
-    const int b = (int)q;
-    return b < N - 1 ? b : N - 1;
+    int b = (int)q;
+    b = b < N - 1 ? b : N - 1;
+    return b > 0 ? b : 0;  // q >= -1 for any finite cube; keeps garbage geometry in range
 }
 
 // The same, plus the clamped relative coordinate 2((x-o)/len - b) - 1
@@ -62,6 +71,7 @@ __device__ __forceinline__ int leaf_coord_rel(T x, T o, T len, int N, T &rel)
     const T q = (x - o) / len;
     int b = (int)q;
     b = b < N - 1 ? b : N - 1;
+    b = b > 0 ? b : 0;
     T r = T(2) * (q - (T)b) - T(1);
     r = r < T(1) ? r : T(1);
     r = r > T(-1) ? r : T(-1);

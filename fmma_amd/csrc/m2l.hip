@@ -117,12 +117,26 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
 void m2l_shape(int P, int mode, int &MP, int &KP)
 {
     MP = (P + 15) / 16 * 16;
-    // beyond 128 the pipelined contraction takes chunks of 32 (FP64, FP32)
-    // or 128 (BF16x3) of the padded length
-    if (mode == 2)
+    // Padded to a length with a pipelined kernel: 32, 48, 64, 96 or 128
+    // (FP64, FP32; 48 and 96 serve 2-D orders such as 6 nodes, P = 36), a
+    // multiple of 32 (BF16x3).  Beyond 128 the pipelined contraction takes
+    // chunks of 32 (FP64, FP32) or 128 (BF16x3) of the padded length.  Up to
+    // 16 coefficients (low orders, 1-D) the length is only rounded to the
+    // MFMA's k-step of 4.
+    if (mode == 2) {
         KP = P <= 128 ? (P + 31) / 32 * 32 : (P + 127) / 128 * 128;
-    else
-        KP = P <= 128 ? (P + 3) / 4 * 4 : (P + 31) / 32 * 32;
+    } else if (P <= 16) {
+        KP = (P + 3) / 4 * 4;
+    } else if (P <= 128) {
+        KP = 128;
+        for (int c : {32, 48, 64, 96})
+            if (P <= c) {
+                KP = c;
+                break;
+            }
+    } else {
+        KP = (P + 31) / 32 * 32;
+    }
 }
 
 size_t m2l_frag_elems(int n_far, int P, int mode)
@@ -643,11 +657,20 @@ static bool launch_pipe_kp(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 template <typename T, int D, int MODE>
 static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 {
-    switch (a.KP) {
-    case 32: return launch_pipe_kp<T, D, MODE, 32>(a, tb, grid, st);
-    case 36: return launch_pipe_kp<T, D, MODE, 36>(a, tb, grid, st);
-    case 64: return launch_pipe_kp<T, D, MODE, 64>(a, tb, grid, st);
-    case 128: return launch_pipe_kp<T, D, MODE, 128>(a, tb, grid, st);
+    if constexpr (D == 1) {
+        return false;  // at most 16 coefficients per box
+    } else {
+        switch (a.KP) {
+        case 32: return launch_pipe_kp<T, D, MODE, 32>(a, tb, grid, st);
+        case 64: return launch_pipe_kp<T, D, MODE, 64>(a, tb, grid, st);
+        case 128: return launch_pipe_kp<T, D, MODE, 128>(a, tb, grid, st);
+        }
+    }
+    if constexpr (D == 2 && MODE != 2) {
+        switch (a.KP) {
+        case 48: return launch_pipe_kp<T, D, MODE, 48>(a, tb, grid, st);
+        case 96: return launch_pipe_kp<T, D, MODE, 96>(a, tb, grid, st);
+        }
     }
     if constexpr (D == 3) {
         // 6, 7 and 8 nodes per dimension
@@ -671,7 +694,7 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 bool m2l_pipe_supported(int D, int KP, int mode, int tb)
 {
     const int vec = mode == 2 ? 8 : mode == 1 ? 4 : 2;
-    const bool small = KP == 32 || KP == 36 || KP == 64 || KP == 128;
+    const bool small = D >= 2 && (KP == 32 || KP == 64 || KP == 128 || (D == 2 && mode != 2 && (KP == 48 || KP == 96)));
     const bool big = D == 3 && (mode == 2 ? (KP == 256 || KP == 384 || KP == 512)
                                           : (KP == 224 || KP == 352 || KP == 512));
     if (!small && !big) return false;

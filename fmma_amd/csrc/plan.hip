@@ -8,6 +8,7 @@
 //   -> L2L (levels 2..L-1) -> L2P -> P2P (writes ans in caller order).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -147,6 +148,7 @@ struct Plan {
     int forced_tb = 0;                    // > 0: this M2L tile width wherever a pipelined kernel has it
     std::map<long, int> tuned_tb;        // batches: (dim, P, mode, parents per level) -> tile width
     std::map<long, int> tuned_tb_single; // single problems: (dim, P, mode, parents per level) -> tile width
+    std::map<long, int> used_tb_single;  // the last single solve's tasks: parents per level -> tile width
     int tuned_set_L = -1, tuned_set_mode = -1;  // the single-problem tasks carry tuned widths
     hipStream_t cap_stream = nullptr;     // private stream for graph capture
 
@@ -227,6 +229,15 @@ static long tune_key(int D, int P, int mode, long npar)
 static int frag_format(int mode)
 {
     return mode == 0 ? FMMA_FRAG_F64 : mode == 1 ? FMMA_FRAG_F32 : FMMA_FRAG_BF16;
+}
+
+// The tile width of each level's tasks (by parents per level), as reported
+// by fmma_plan_m2l_tiles for single solves.
+static void record_tiles(Plan &pl, const std::vector<M2LTask> &tasks, int D)
+{
+    pl.used_tb_single.clear();
+    for (const M2LGroup &g : pl.groups)
+        for (int i = g.first; i < g.first + g.count; ++i) pl.used_tb_single[1L << (D * (tasks[i].level - 1))] = g.tb;
 }
 
 // Constant tables that depend on (poly_ord, dim, dtype): nodes, Chebyshev
@@ -320,6 +331,7 @@ static int setup(Plan &pl, int L)
             if ((rc = pl.Wc.ensure(offc * opsz))) return rc;
             if (mode == 2 && (rc = pl.Wc_lo.ensure(offc * 2))) return rc;
             std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off);
+            record_tiles(pl, tasks, D);
             std::vector<int> op_levels, op_prob(L - 1, 0);
             for (int l = 2; l <= L; ++l) op_levels.push_back(l);
             if ((rc = upload(pl.d_op_levels, op_levels.data(), op_levels.size() * sizeof(int), st))) return rc;
@@ -335,6 +347,27 @@ static int setup(Plan &pl, int L)
         pl.tuned_set_L = -1;
     }
     return FMMA_OK;
+}
+
+// Reads back the bounding cube just computed and rejects inputs the
+// reference cannot bin (Geom::status), before any kernel uses the cube.
+static int geom_status_error(int status)
+{
+    if (status == kGeomNonFinite)
+        return fail(FMMA_ERR_INVALID, "a target or source coordinate is NaN or infinite");
+    if (status == kGeomZeroLen)
+        return fail(FMMA_ERR_INVALID, "every target and source is at the same position: the bounding cube has "
+                                      "size 0 and its boxes are undefined");
+    return FMMA_OK;
+}
+
+template <typename T>
+static int check_geom(Plan &pl)
+{
+    Geom<T> g;
+    FMMA_HIP(hipMemcpyAsync(&g, pl.geom.p, sizeof(g), hipMemcpyDeviceToHost, pl.stream));
+    FMMA_HIP(hipStreamSynchronize(pl.stream));
+    return geom_status_error(g.status);
 }
 
 // Sort one point set into its leaf boxes: keys -> radix sort -> ranges ->
@@ -426,6 +459,7 @@ static int tune_single(Plan &pl, const M2LKArgs &ka, int L)
         tb_of[l] = best_tb;
     }
     std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off, tb_of);
+    record_tiles(pl, tasks, D);
     pl.n_tasks = (int)tasks.size();
     if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
     FMMA_HIP(hipStreamSynchronize(st));
@@ -451,9 +485,10 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     const unsigned nbox = 1u << (D * L);
 
     pl.mark(0);
-    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * kBBoxStride<D> * sizeof(T))))
         return rc;
     if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
+    if ((rc = check_geom<T>(pl))) return rc;
     if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
                                pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start)))
         return rc;
@@ -750,9 +785,10 @@ static int run_tree(Plan &pl, long nt, const T *tg, const T *w, long ns, const T
     const unsigned nbox = 1u << (D * L);
 
     pl.mark(0);
-    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * kBBoxStride<D> * sizeof(T))))
         return rc;
     if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
+    if ((rc = check_geom<T>(pl))) return rc;
     const T *t_pos;
     const unsigned *t_start, *t_perm;
     if ((rc = sort_points<T, D>(pl, nt, tg, w, ns, sr, L, 0, t_pos, t_start, t_perm))) return rc;
@@ -805,9 +841,10 @@ static int run_nrnmm(Plan &pl, long nt, const T *tg, const T *w, long ns, const 
     const bool shared = (const void *)tg == (const void *)sr && nt == ns;
 
     pl.mark(0);
-    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * 2 * D * sizeof(T))))
+    if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * kBBoxStride<D> * sizeof(T))))
         return rc;
     if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), g, st))) return rc;
+    if ((rc = check_geom<T>(pl))) return rc;
     const T *t_pos;
     const unsigned *t_start, *t_perm;
     if ((rc = sort_points<T, D>(pl, nt, tg, w, ns, sr, 0, g, t_pos, t_start, t_perm))) return rc;
@@ -1427,6 +1464,21 @@ static int run_batch(Plan &pl, int nprob, const long *tc, const void *tg, const 
     }
     if (rc) return rc;
     FMMA_HIP(hipStreamSynchronize(pl.stream));
+    // The batch runs as one captured sequence, so the cubes are checked after
+    // it: a problem the reference cannot bin fails the call (ans is then
+    // undefined; the kernels clamp box coordinates and stay in bounds).
+    {
+        const size_t gsz = pl.dtype == FMMA_F64 ? sizeof(Geom<double>) : sizeof(Geom<float>);
+        const size_t soff = pl.dtype == FMMA_F64 ? offsetof(Geom<double>, status) : offsetof(Geom<float>, status);
+        std::vector<unsigned char> hg((size_t)nprob * gsz);
+        FMMA_HIP(hipMemcpy(hg.data(), pl.batch->d_geoms.p, hg.size(), hipMemcpyDeviceToHost));
+        for (int p = 0; p < nprob; ++p) {
+            int status;
+            std::memcpy(&status, hg.data() + (size_t)p * gsz + soff, sizeof(int));
+            if ((rc = geom_status_error(status)))
+                return fail(rc, "batch problem " + std::to_string(p) + ": " + g_err);
+        }
+    }
     if (pl.timing && pl.events_ready) {
         float total = 0.f;
         FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
@@ -1643,7 +1695,7 @@ int fmma_plan_m2l_tiles(fmma_plan plan, int batch, long *parents, int *tile, int
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     const Plan &pl = plan->pl;
     int i = 0;
-    for (const auto &kv : batch ? pl.tuned_tb : pl.tuned_tb_single) {
+    for (const auto &kv : batch ? pl.tuned_tb : pl.used_tb_single) {
         if (i < max && parents && tile) {
             parents[i] = kv.first & 0xffffffffL;
             tile[i] = kv.second;

@@ -14,9 +14,12 @@ namespace lib {
 
 // ------------------------------------------------------------ bounding cube
 
+// Partial min/max per block, and whether the block saw a non-finite
+// coordinate (min/max alone would skip a NaN): kBBoxStride<D> values per block.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T *b, long nb, T *part)
 {
+    bool bad = false;
     T lo[D], hi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -30,6 +33,7 @@ __global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T
             const T v = a[i * D + d];
             lo[d] = v < lo[d] ? v : lo[d];
             hi[d] = hi[d] < v ? v : hi[d];
+            bad = bad || !__builtin_isfinite(v);
         }
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nb; i += stride)
 #pragma unroll
@@ -37,7 +41,12 @@ __global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T
             const T v = b[i * D + d];
             lo[d] = v < lo[d] ? v : lo[d];
             hi[d] = hi[d] < v ? v : hi[d];
+            bad = bad || !__builtin_isfinite(v);
         }
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    if (bad) s_bad = 1;
 #pragma unroll
     for (int d = 0; d < D; ++d)
         for (int o = 32; o > 0; o >>= 1) {
@@ -62,7 +71,8 @@ __global__ __launch_bounds__(256) void bbox_partial(const T *a, long na, const T
                 s[0][D + d] = s[0][D + d] < s[w][D + d] ? s[w][D + d] : s[0][D + d];
             }
 #pragma unroll
-        for (int d = 0; d < 2 * D; ++d) part[blockIdx.x * 2 * D + d] = s[0][d];
+        for (int d = 0; d < 2 * D; ++d) part[blockIdx.x * kBBoxStride<D> + d] = s[0][d];
+        part[blockIdx.x * kBBoxStride<D> + 2 * D] = s_bad ? T(1) : T(0);
     }
 }
 
@@ -73,18 +83,22 @@ template <typename T, int D>
 __global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> *g, int n_side)
 {
     T lo[D], hi[D];
+    bool bad = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         lo[d] = T(INFINITY);
         hi[d] = T(-INFINITY);
     }
-    for (int p = threadIdx.x; p < np; p += 64)
+    for (int p = threadIdx.x; p < np; p += 64) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const T l = part[p * 2 * D + d], h = part[p * 2 * D + D + d];
+            const T l = part[p * kBBoxStride<D> + d], h = part[p * kBBoxStride<D> + D + d];
             lo[d] = l < lo[d] ? l : lo[d];
             hi[d] = hi[d] < h ? h : hi[d];
         }
+        bad = bad || part[p * kBBoxStride<D> + 2 * D] != T(0);
+    }
+    bad = __any(bad);
 #pragma unroll
     for (int d = 0; d < D; ++d)
         for (int o = 32; o > 0; o >>= 1) {
@@ -105,6 +119,7 @@ __global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> 
         for (int d = D; d < 3; ++d) g->origin[d] = T(0);
         g->Len = Len;
         g->len_leaf = Len / (T)n_side;
+        g->status = bad ? kGeomNonFinite : Len == T(0) ? kGeomZeroLen : kGeomOk;
     }
 }
 

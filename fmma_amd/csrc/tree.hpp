@@ -6,6 +6,8 @@ namespace fmma {
 namespace lib {
 
 constexpr int kBBoxParts = 1024;  // partial min/max blocks of the bounding-cube reduction
+template <int D>
+constexpr int kBBoxStride = 2 * D + 1;  // per block: min[D], max[D], non-finite flag
 
 struct SortBuffers {
     unsigned *keys;      // [n] in: leaf keys; out: sorted keys

@@ -33,7 +33,31 @@
 
 #include "../fmma_hip.h"
 
+#if defined(__HIPCC__)
+#define FMMA_HOST_DEVICE __host__ __device__
+#else
+#define FMMA_HOST_DEVICE
+#endif
+
 namespace fmma {
+
+// Elementwise +, -, *, / of two points, as the reference defines them for
+// kernels written over std::array (include/fmma/fmma.hpp:15-49); usable in
+// host code and, compiled with hipcc, inside a device functor.
+#define FMMA_POINT_OPERATOR(OP)                                                                             \
+    template <typename TYPE, std::size_t DIM>                                                               \
+    FMMA_HOST_DEVICE inline std::array<TYPE, DIM> operator OP(const std::array<TYPE, DIM> &a,              \
+                                                             const std::array<TYPE, DIM> &b)               \
+    {                                                                                                       \
+        std::array<TYPE, DIM> r{};                                                                          \
+        for (std::size_t d = 0; d < DIM; ++d) r[d] = a[d] OP b[d];                                          \
+        return r;                                                                                           \
+    }
+FMMA_POINT_OPERATOR(+)
+FMMA_POINT_OPERATOR(-)
+FMMA_POINT_OPERATOR(*)
+FMMA_POINT_OPERATOR(/)
+#undef FMMA_POINT_OPERATOR
 
 template <typename TYPE, std::size_t DIM>
 class FMMA {

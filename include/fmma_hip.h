@@ -367,10 +367,11 @@ int fmma_plan_set_m2l_autotune(fmma_plan plan, int enable);  /* default 1 */
  * where a pipelined kernel has the width (the rule's elsewhere), for tests and
  * measurements.  No reference counterpart. */
 int fmma_plan_set_m2l_tile(fmma_plan plan, int tile);
-/* The M2L tile autotuner's choices so far, for batches (batch = 1) or single
- * solves (0): tile width (target boxes per workgroup) per number of parent
- * boxes of a level.  Returns the number of entries.  The single-solve tuner
- * runs on the first fmm solve of a (Depth, M2L form) when autotuning is on. */
+/* M2L tile widths (target boxes per workgroup) per number of parent boxes of
+ * a level: the autotuner's choices so far for batches (batch = 1), or the
+ * widths the last single fmm solve's tasks use (0).  Returns the number of
+ * entries.  The single-solve tuner runs on the first fmm solve of a (Depth,
+ * M2L form) when autotuning is on. */
 int fmma_plan_m2l_tiles(fmma_plan plan, int batch, long *parents, int *tile, int max);
 
 /* Per-stage milliseconds, accumulated over solves like the reference's

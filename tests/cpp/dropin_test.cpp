@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <stdexcept>
 #include <vector>
 
 static uint64_t sm_state = 0;
@@ -240,6 +241,50 @@ int main()
         check("nrnmm_lambda_vs_builtin", rel(na, nb), 1e-12);
         check("tree_vs_exact", rel(ta, e), 1e-3);
         check("nrnmm_vs_exact", rel(na, e), 1e-3);
+    }
+    {  // a two-argument kernel written with the point operators of fmma.hpp
+       // (include/fmma/fmma.hpp:15-49), in device code, vs the built-in 1/r
+        const size_t n = 6000;
+        auto x = pts<double, 3>(n);
+        std::vector<double> w(n);
+        for (auto &v : w) v = uni();
+        fmma::FMMA<double, 3> f, g;
+        f.set_fn([] __host__ __device__(const std::array<double, 3> &a, const std::array<double, 3> &b) {
+            using fmma::operator-;
+            using fmma::operator*;
+            const std::array<double, 3> d = a - b, q = d * d;
+            const double r2 = q[0] + q[1] + q[2];
+            return r2 == 0.0 ? 0.0 : 1.0 / std::sqrt(r2);
+        });
+        g.set_kernel(FMMA_KERNEL_INV_R_REG);
+        std::vector<double> a, b;
+        for (auto *p : {&f, &g}) {
+            p->set_solver_type("fmm");
+            p->set_poly_ord(3);
+            p->set_Depth(3);
+            p->set_trans_sym_flag(1);
+        }
+        f.solve(x, w, x, a);
+        g.solve(x, w, x, b);
+        check("point_operators_vs_builtin", rel(a, b), 1e-12);
+        using fmma::operator+;
+        using fmma::operator/;
+        const std::array<double, 2> u{1.0, 6.0}, v{3.0, 2.0}, s = u + v, r = u / v;
+        check("point_operators_host", std::abs(s[0] - 4.0) + std::abs(s[1] - 8.0) + std::abs(r[1] - 3.0), 0);
+    }
+    {  // every point at one position: the cube has size 0 and the solve is rejected
+        std::vector<std::array<double, 3>> x(100, std::array<double, 3>{0.5, 0.5, 0.5});
+        std::vector<double> w(100, 1.0), a;
+        fmma::FMMA<double, 3> f;
+        f.set_solver_type("fmm");
+        f.set_poly_ord(2);
+        bool rejected = false;
+        try {
+            f.solve(x, w, x, a);
+        } catch (const std::runtime_error &) {
+            rejected = true;
+        }
+        check("zero_size_cube_rejected", rejected ? 0 : 1, 0);
     }
     {  // time log keys of the reference (src/fmm.cpp:57-141, src/fmma.cpp:189)
         fmma::FMMA<double, 2> f;

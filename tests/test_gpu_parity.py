@@ -244,8 +244,9 @@ def test_repeated_solves_are_deterministic(gpu):
 
 @pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
 def test_m2l_forms_at_512_coefficients(gpu, oracle, prec, tol):
-    """8 nodes per dimension in 3-D (P = 512): the contraction of the
-    non-pipelined M2L kernels takes several LDS passes."""
+    """8 nodes per dimension in 3-D (P = 512): the pipelined M2L kernels cut
+    each offset's contraction into chunks (32 for FP64/FP32, 128 for BF16x3)
+    at 64-box tiles."""
     x = synth.points(75, 3000, 3)
     w = synth.normal(76, 3000)
     got, pl = gpu_solve(gpu, "fmm", x, w, x, 7, 3, 1, "inv_r_reg", shared=True, precision=prec)
@@ -277,3 +278,69 @@ def test_m2l_every_tile_width(gpu, oracle, p, prec, tol, tile):
     err = rel_l2(got, _TILE_REF[p])
     print("  p=%d %s tile %d: %.3e, tiles %s" % (p, prec, tile, err, pl.m2l_tiles(batch=False)))
     assert err <= tol
+
+
+_FULL_REF = {}
+
+
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-6)])
+@pytest.mark.parametrize("tile", [128, 256])
+def test_m2l_full_tiles_depth5(gpu, oracle, prec, tol, tile):
+    """Depth 5 (512 parents at level 4 of 8^4 boxes): the 128- and 256-box
+    tiles of the pipelined M2L are full, as at N = 4M, Depth 6."""
+    n = 30000
+    x = synth.points(97, n, 3)
+    w = synth.uniform(98, n)
+    if "ref" not in _FULL_REF:
+        _FULL_REF["ref"] = oracle.fmm(x, w, x, 3, 5, 1, KID["inv_r_reg"])
+    pl = gpu.Plan(3, np.float64)
+    pl.set_Depth(5)
+    pl.set_m2l_tile(tile)
+    got, _ = gpu_solve(gpu, "fmm", x, w, x, 3, 5, 1, "inv_r_reg", shared=True, precision=prec, plan=pl)
+    tiles = pl.m2l_tiles(batch=False)
+    err = rel_l2(got, _FULL_REF["ref"])
+    print("  depth 5 %s tile %d: %.3e, tiles %s" % (prec, tile, err, tiles))
+    if not (prec == "native" and tile == 256):  # no 256-box FP64 kernel: the tuner's width
+        assert tiles.get(512) == tile, tiles
+    assert err <= tol, err
+
+
+_2D_REF = {}
+
+
+@pytest.mark.parametrize("p,depth,n", [(5, 5, 12000), (5, 6, 40000), (3, 5, 12000), (7, 5, 12000)])
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 1e-5)])
+def test_m2l_forms_2d_log(gpu, oracle, p, depth, n, prec, tol):
+    """The 2-D precision forms (config 2's kernel, log|x-y| with 0 at r = 0,
+    targets = sources, weights N(0,1)): 6 nodes per dimension (P = 36, padded
+    to 48 / 64 for the pipelined kernels), 4 nodes (P = 16: the non-pipelined
+    kernels) and 8 nodes (P = 64), against the oracle.
+
+    BF16x3's bar is 1e-5: its two-term BF16 splits carry about 17 significant
+    bits per operand (FP32: 24), and each log|x-y| operator is dominated by
+    its constant part log(distance), whose contributions cancel in
+    sum_j w_j log|x-y| over weights of both signs; measured 3.5e-6 at 6 nodes,
+    Depth 5."""
+    key = (p, depth, n)
+    x = synth.points(101, n, 2)
+    w = synth.normal(102, n)
+    if key not in _2D_REF:
+        _2D_REF[key] = oracle.fmm(x, w, x, p, depth, 1, KID["log_r"])
+    got, pl = gpu_solve(gpu, "fmm", x, w, x, p, depth, 1, "log_r", shared=True, precision=prec)
+    err = rel_l2(got, _2D_REF[key])
+    print("  2-D p=%d depth %d %s: %.3e, tiles %s" % (p, depth, prec, err, pl.m2l_tiles(batch=False)))
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-6)])
+def test_m2l_forms_1d(gpu, oracle, prec, tol):
+    """Config 1 (1-D, 1/(x-y), 4096 independent targets and sources, weights
+    U[-1,1], 8 nodes, auto Depth 9) in every M2L form against the oracle."""
+    t = synth.points(11, 4096, 1)
+    s = synth.points(12, 4096, 1)
+    w = 2 * synth.uniform(13, 4096) - 1
+    got, pl = gpu_solve(gpu, "fmm", t, w, s, 7, -1, 1, "inv_diff", precision=prec)
+    ref = oracle.fmm(t, w, s, 7, pl.depth, 1, KID["inv_diff"])
+    err = rel_l2(got, ref)
+    print("  1-D %s depth %d: %.3e" % (prec, pl.depth, err))
+    assert err <= tol, err
