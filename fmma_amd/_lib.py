@@ -38,7 +38,7 @@ EXPORTS = [
     "fmma_plan_set_kernel", "fmma_plan_set_kernel_ops", "fmma_plan_set_m2l_precision", "fmma_plan_set_stream",
     "fmma_plan_set_timing", "fmma_plan_set_p2p_mutual", "fmma_plan_gradient", "fmma_plan_gradient_host",
     "fmma_plan_solve_transpose", "fmma_plan_solve_transpose_host", "fmma_plan_solve_batch",
-    "fmma_plan_set_batch_graph", "fmma_plan_set_m2l_autotune", "fmma_plan_set_m2l_tile", "fmma_plan_m2l_tiles", "fmma_plan_solve", "fmma_plan_solve_host", "fmma_plan_time_log",
+    "fmma_plan_set_batch_graph", "fmma_plan_set_m2l_autotune", "fmma_plan_set_m2l_tile", "fmma_plan_m2l_tiles", "fmma_plan_solve", "fmma_plan_solve_noalloc", "fmma_plan_solve_host", "fmma_plan_time_log",
     "fmma_plan_clear_time_log", "fmma_plan_level_coeffs", "fmma_plan_point_boxes", "fmma_morton_to_rowmajor",
     "fmma_last_error",
     "fmma_version",
@@ -90,6 +90,7 @@ def lib():
     L.fmma_plan_set_m2l_tile.argtypes = [vp, i]
     L.fmma_plan_m2l_tiles.argtypes = [vp, i, lp, ctypes.POINTER(ctypes.c_int), i]
     L.fmma_plan_solve.argtypes = [vp, l, vp, vp, l, vp, vp]
+    L.fmma_plan_solve_noalloc.argtypes = [vp, l, vp, vp, l, vp, vp]
     L.fmma_plan_solve_host.argtypes = [vp, l, vp, vp, l, vp, vp]
     L.fmma_plan_time_log.argtypes = [vp, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_double), i]
     L.fmma_plan_clear_time_log.argtypes = [vp]

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -41,15 +42,37 @@ static int fail(int code, const std::string &msg)
     return code;
 }
 
+// ------------------------------------------------------------ no-alloc scratch
+//
+// fmma_plan_solve_noalloc takes every device buffer of its solve from
+// `scratch`, one static array per GPU that lives until the process ends, so
+// that it calls no hipMalloc / hipFree.  Its device address comes from
+// hipGetSymbolAddress, once per device with that device current (host code
+// never passes the array's name to a kernel).  Calls that overlap on one GPU
+// share the array and race; each call initialises everything it reads.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The region a no-alloc solve carves its buffers from (bump allocation),
+// active on the calling thread for the duration of that solve.
+struct Arena {
+    char *base;
+    size_t cap, used;
+};
+static thread_local Arena *t_arena = nullptr;
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool borrowed = false;  // carved from the no-alloc scratch: never freed
     int ensure(size_t bytes)
     {
+        if (t_arena) return carve(bytes);
         if (bytes <= cap) return FMMA_OK;
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        borrowed = false;
         const hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
@@ -58,11 +81,32 @@ struct DBuf {
         cap = bytes;
         return FMMA_OK;
     }
+    int carve(size_t bytes)
+    {
+        if (borrowed && bytes <= cap) return FMMA_OK;
+        if (p && !borrowed) {
+            set_error("internal: a buffer of the no-alloc solve was allocated before it");
+            return FMMA_ERR_INVALID;
+        }
+        const size_t off = (t_arena->used + 255) & ~size_t(255);
+        if (off > t_arena->cap || bytes > t_arena->cap - off) {
+            set_error("fmma_plan_solve_noalloc: this solve needs more than the " +
+                      std::to_string(kNoAllocScratchBytes >> 20) + " MiB of no-alloc scratch (" +
+                      std::to_string((off + bytes + (1 << 20) - 1) >> 20) +
+                      " MiB or more); use fmma_plan_solve or a smaller problem");
+            return FMMA_ERR_SIZE;
+        }
+        p = t_arena->base + off;
+        cap = bytes;
+        borrowed = true;
+        t_arena->used = off + bytes;
+        return FMMA_OK;
+    }
     template <class X>
     X *as() const { return static_cast<X *>(p); }
     ~DBuf()
     {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
     }
 };
 
@@ -318,7 +362,6 @@ static int setup(Plan &pl, int L)
             for (int l = 0; l < kMaxLevels; ++l) pl.kf_off[l] = l >= 2 ? (long)(l - 2) * (long)fe : 0;
             if ((rc = pl.Kf.ensure(fe * esz * (L - 1)))) return rc;
             if (mode == 2 && (rc = pl.Kf_lo.ensure(fe * esz * (L - 1)))) return rc;
-            if ((rc = pl.Kraw.ensure((size_t)pl.offs.n_far * P * P * pl.tsize()))) return rc;
             // padded operand rows of KP for levels 2..L
             int MP, KP;
             m2l_shape(P, mode, MP, KP);
@@ -543,6 +586,7 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
             rc = pl.ops.m2l_op(&oa, pl.ops.ctx, st);
             if (rc == FMMA_ERR_UNSUPPORTED) {
                 // a launcher table without the fragment form: one level at a time through Kraw
+                if ((rc = pl.Kraw.ensure((size_t)pl.offs.n_far * P * P * pl.tsize()))) return rc;
                 for (int l = 2; l <= L; ++l) {
                     fmma_m2l_op_args ra{};
                     ra.dim = D;
@@ -1492,6 +1536,72 @@ static int solve_device(Plan &pl, long nt, const void *tg, const void *w, long n
     return run_op(pl, Op::Solve, nt, tg, w, ns, sr, ans, false);
 }
 
+// This GPU's scratch address, from hipGetSymbolAddress on the first no-alloc
+// call with that device current.
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;  // per device
+
+static int scratch_ptr(void **out)
+{
+    int dev = 0;
+    FMMA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        FMMA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(FMMA_ERR_HIP, "hipGetSymbolAddress gave a null address for the no-alloc scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FMMA_OK;
+}
+
+struct ArenaScope {
+    explicit ArenaScope(Arena *a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
+// fmma_plan_solve with every device buffer carved from the scratch: a
+// private plan with the caller's settings (kernel, solver, order, Depth,
+// trans_sym, M2L form, stream) runs the solve, with the rule's M2L tile
+// widths (or the forced one) instead of the autotuner and without the stage
+// timing, so that it creates no events and allocates no device memory.  The
+// auto Depth / nrn_N it picks is kept by the caller's plan, as a solve would.
+static int solve_noalloc(Plan &user, long nt, const void *tg, const void *w, long ns, const void *sr, void *ans)
+{
+    void *base = nullptr;
+    int rc;
+    if ((rc = scratch_ptr(&base))) return rc;
+    std::unique_ptr<Plan> tmp(new Plan());
+    Plan &pl = *tmp;
+    pl.dim = user.dim;
+    pl.dtype = user.dtype;
+    pl.solver = user.solver;
+    pl.poly_ord = user.poly_ord;
+    pl.depth = user.depth;
+    pl.trans_sym = user.trans_sym;
+    pl.nrn_N = user.nrn_N;
+    pl.kernel_id = user.kernel_id;
+    pl.kparams = user.kparams;
+    pl.ops = user.ops;  // a built-in's ctx points at user.kparams, alive for this call
+    pl.precision = user.precision;
+    pl.stream = user.stream;
+    pl.p2p_mutual = user.p2p_mutual;
+    pl.forced_tb = user.forced_tb;
+    pl.timing = false;
+    pl.m2l_autotune = false;
+    Arena ar{static_cast<char *>(base), kNoAllocScratchBytes, 0};
+    {
+        ArenaScope scope(&ar);
+        rc = run_op(pl, Op::Solve, nt, tg, w, ns, sr, ans, false);
+    }
+    if (rc) (void)hipStreamSynchronize(pl.stream);  // no kernel of this call still uses the scratch
+    if (user.depth <= 0 && pl.depth > 0 && (pl.solver == "fmm" || pl.solver == "tree")) user.depth = pl.depth;
+    if (user.nrn_N <= 0 && pl.nrn_N > 0 && pl.solver == "nrnmm") user.nrn_N = pl.nrn_N;
+    return rc;
+}
+
 }  // namespace lib
 }  // namespace fmma
 
@@ -1759,6 +1869,13 @@ static int run_op_host(Plan &pl, fmma::lib::Op op, long n_t, const void *targets
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fmma::lib::hip_fail(e, "copy the result to the host");
     return FMMA_OK;
+}
+
+int fmma_plan_solve_noalloc(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,
+                            const void *sources, void *ans)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    FMMA_GUARD(return fmma::lib::solve_noalloc(plan->pl, n_t, targets, weights, n_s, sources, ans));
 }
 
 int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights, long n_s,

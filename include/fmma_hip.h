@@ -311,6 +311,22 @@ int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable);
 int fmma_plan_solve(fmma_plan plan, long n_t, const void *targets, const void *weights,
                     long n_s, const void *sources, void *ans);
 
+/* No counterpart in the reference: fmma_plan_solve for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  The same solve, arguments and results (to rounding), but every
+ * device buffer it needs (tree, multipoles, operators, near-field slots) is
+ * carved from one static 64 MiB scratch array per GPU that the library holds
+ * until the process ends; it calls no hipMalloc or hipFree and creates no
+ * events, so it records no stage times, and it uses the rule's M2L tile
+ * widths (or fmma_plan_set_m2l_tile's) instead of the autotuner.  A solve
+ * that needs more than the scratch (3-D FP64 at 4 nodes per dimension:
+ * beyond about 150,000 points) fails with FMMA_ERR_SIZE before ans is
+ * written.  Calls that overlap on one GPU share its scratch and race; use one
+ * stream (or serialise the calls).  The auto Depth / nrn_N it picks is kept
+ * by the plan, as with fmma_plan_solve. */
+int fmma_plan_solve_noalloc(fmma_plan plan, long n_t, const void *targets, const void *weights,
+                            long n_s, const void *sources, void *ans);
+
 /* The same with HOST pointers: the library stages them through device
  * memory (the copies are outside the per-stage time log). */
 int fmma_plan_solve_host(fmma_plan plan, long n_t, const void *targets, const void *weights,
