@@ -189,6 +189,7 @@ struct Plan {
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
     std::unique_ptr<BatchPlan> batch;
     bool batch_graph = true, m2l_autotune = true;
+    bool batch_eager = false;             // the last batched solve ran eagerly (stage marks recorded)
     int forced_tb = 0;                    // > 0: this M2L tile width wherever a pipelined kernel has it
     std::map<long, int> tuned_tb;        // batches: (dim, P, mode, parents per level) -> tile width
     std::map<long, int> tuned_tb_single; // single problems: (dim, P, mode, parents per level) -> tile width
@@ -205,6 +206,7 @@ struct Plan {
     long lvl_off[kMaxLevels + 1] = {};
     long kf_off[kMaxLevels] = {};
     int last_L = -1, last_shared = 0;
+    bool last_m2l_general = false;  // the last fmm passes evaluated f per box pair in M2L
     long last_nt = 0, last_ns = 0;
 
     // The tree and local expansions of the last fmm passes, for a gradient
@@ -405,12 +407,33 @@ static int geom_status_error(int status)
 }
 
 template <typename T>
-static int check_geom(Plan &pl)
+static int check_geom(Plan &pl, Geom<T> *out = nullptr)
 {
     Geom<T> g;
     FMMA_HIP(hipMemcpyAsync(&g, pl.geom.p, sizeof(g), hipMemcpyDeviceToHost, pl.stream));
     FMMA_HIP(hipStreamSynchronize(pl.stream));
+    if (out) *out = g;
     return geom_status_error(g.status);
+}
+
+// With trans_sym_flag 0 the reference evaluates f at absolute node positions
+// len*(b + (1+c)/2) + origin (src/fmm_core.cpp:466-487); per-offset
+// operators use the relative ones of flag 1 (src/fmm_core.cpp:407-408).  For
+// a translation-invariant f the two differ only by the rounding of the
+// absolute positions, about eps * |position| / len_leaf relative: measured
+// 2.4e-17 times that ratio in FP64 (4.0e-12 for [1e5, 1e5 + 1)^3 at leaf
+// level 3).  Past these ratios (FP64: about 1e-13; FP32: a tenth of its
+// 3e-5 bar) a flag-0 solve evaluates f per box pair as the reference does.
+template <typename T>
+static bool absolute_positions_matter(const Geom<T> &g, int D, int L)
+{
+    const double limit = sizeof(T) == 8 ? 4096.0 : 256.0;
+    double R = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double o = (double)g.origin[d];
+        R = std::max(R, std::max(std::fabs(o), std::fabs(o + (double)g.Len)));
+    }
+    return R * std::ldexp(1.0, L) > limit * (double)g.Len;
 }
 
 // Sort one point set into its leaf boxes: keys -> radix sort -> ranges ->
@@ -531,7 +554,8 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     if ((rc = pl.geom.ensure(sizeof(Geom<T>))) || (rc = pl.bbox_part.ensure(kBBoxParts * kBBoxStride<D> * sizeof(T))))
         return rc;
     if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
-    if ((rc = check_geom<T>(pl))) return rc;
+    Geom<T> hgeom;
+    if ((rc = check_geom<T>(pl, &hgeom))) return rc;
     if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
                                pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start)))
         return rc;
@@ -561,7 +585,9 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     pl.mark(3);
 
     if (L >= 2) {
-        const bool sym = pl.trans_sym == 1 || pl.ops.translation_invariant;
+        const bool sym =
+            pl.trans_sym == 1 || (pl.ops.translation_invariant && !absolute_positions_matter(hgeom, D, L));
+        pl.last_m2l_general = !sym;
         if (sym) {
             // the operators of levels 2..L straight into fragment order
             fmma_m2l_op_args oa{};
@@ -1449,9 +1475,11 @@ static int batch_typed(Plan &pl, int nprob, const long *tc, const void *tg_, con
             FMMA_HIP(hipStreamSynchronize(pl.stream));
             bp.tuned = true;
         } else {
+            pl.batch_eager = true;
             return batch_kernels<T, D>(pl, bp, tg, w, sr, ans, pl.stream, true, true);
         }
     }
+    pl.batch_eager = !pl.batch_graph;
     if (!pl.batch_graph) return batch_kernels<T, D>(pl, bp, tg, w, sr, ans, pl.stream, true, false);
     // the graph holds every buffer address: capture again when any moved
     const std::vector<const void *> bufs = batch_buffer_ptrs(pl, tg_, w_, sr_, ans_);
@@ -1527,6 +1555,13 @@ static int run_batch(Plan &pl, int nprob, const long *tc, const void *tg, const 
         float total = 0.f;
         FMMA_HIP(hipEventElapsedTime(&total, pl.ev[0], pl.ev[kStages]));
         pl.time_log["total time(fmm batch)"] = total;
+        if (pl.batch_eager) {  // an eager pass marked its stages (a graph replay does not)
+            for (int i = 0; i < kStages; ++i) {
+                float ms = 0.f;
+                FMMA_HIP(hipEventElapsedTime(&ms, pl.ev[i], pl.ev[i + 1]));
+                pl.time_log[std::string("batch ") + kStageNames[i]] += ms;
+            }
+        }
     }
     return FMMA_OK;
 }

@@ -62,6 +62,23 @@ def batch_points(seed, counts, dim):
     return points(seed, int(np.sum(counts)), dim)
 
 
+def sample2_inputs(sizes):
+    """The inputs of the reference's 2-D benchmark (benchmark/src/sample2.cpp):
+    glibc rand() from its default seed, for N = sizes[0], sizes[1], ... in
+    turn; per point source[n][0], target[n][0], source[n][1], target[n][1],
+    each rand()/RAND_MAX.  Returns {N: (source [N,2], target [N,2])}."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    out = {}
+    for n in sizes:
+        v = np.fromiter((libc.rand() for _ in range(4 * n)), dtype=np.float64, count=4 * n) / 2147483647.0
+        v = v.reshape(n, 2, 2)  # [point][dim][source, target]
+        out[n] = (np.ascontiguousarray(v[:, :, 0]), np.ascontiguousarray(v[:, :, 1]))
+    return out
+
+
 # The five benchmark workloads, as (name, description) -- parameters follow
 # the task's configuration list; bench.py and the tests build them from these.
 def config(name, n=None):

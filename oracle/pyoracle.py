@@ -91,6 +91,10 @@ def ref():
                                     ctypes.c_int, _dp, ctypes.c_long, _fp, ctypes.c_long, _fp, _fp, _fp]
         R.ref_levels_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _dp, ctypes.c_long, _dp, ctypes.c_long, _dp, _dp, _dp, _dp]
+        R.ref_leaf_boxes_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp, ctypes.c_long, _dp,
+                                         ctypes.c_long, _dp, _lp]
+        R.ref_leaf_boxes_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _fp, ctypes.c_long, _fp,
+                                         ctypes.c_long, _fp, _lp]
         R.ref_set_threads.argtypes = [ctypes.c_int]
         R.ref_set_nrn_N.argtypes = [ctypes.c_int]
         _ref = R
@@ -340,6 +344,22 @@ def ref_levels(targets, weights, sources, poly_ord, depth, trans_sym=0, kid=0, p
     ref().ref_levels_f64(dim, poly_ord, depth, trans_sym, kid, _d(p), t.shape[0], _d(t), s.shape[0], _d(w),
                          _d(s), _d(Wm), _d(Wl))
     return split_levels(Wm, dim, P, depth), split_levels(Wl, dim, P, depth)
+
+
+def ref_leaf_boxes(targets, sources, points, depth):
+    """The reference's own row-major leaf box of each point: P2M's binning
+    (src/fmm_core.cpp:255-267) of `points` in the cube of targets and sources."""
+    f32 = np.asarray(points).dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    t, s, p = _c(targets, dt), _c(sources, dt), _c(points, dt)
+    box = np.zeros(p.shape[0], dtype=np.int64)
+    fn = ref().ref_leaf_boxes_f32 if f32 else ref().ref_leaf_boxes_f64
+    cast = (lambda a: a.ctypes.data_as(_fp)) if f32 else _d
+    rc = fn(p.shape[1], int(depth), t.shape[0], cast(t), s.shape[0], cast(s), p.shape[0], cast(p),
+            box.ctypes.data_as(_lp))
+    if rc != 0 or np.any(box < 0):
+        raise RuntimeError("reference leaf boxes failed")
+    return box
 
 
 def ref_set_threads(n):

@@ -148,6 +148,28 @@ struct Stages : public fmma::FMMA<T, D> {
         }
         return 0;
     }
+
+    // The reference's own leaf box of each of `pts` (row-major index): P2M's
+    // binning (src/fmm_core.cpp:255-267) applied to pts as sources, with the
+    // cube of targets and sources (get_origin_and_length, src/fmm_core.cpp:160-173).
+    int leaf_boxes(int depth, long nt, const T* t, long ns, const T* s, long np, const T* pts, long* out)
+    {
+        this->set_poly_ord(1);
+        auto tv = to_aos<T, D>(nt, t), sv = to_aos<T, D>(ns, s), pv = to_aos<T, D>(np, pts);
+        std::array<T, D> origin;
+        T Len;
+        this->get_origin_and_length(tv, sv, origin, Len);
+        std::vector<T> wv(np, T(0));
+        std::vector<std::vector<T>> Wm;
+        std::vector<std::vector<std::size_t>> lists;
+        std::vector<std::array<T, D>> nodes;
+        const std::size_t N = std::size_t(1) << (depth - 1);
+        this->P2M(wv, pv, N, origin, Len / N, lists, Wm, nodes);
+        for (long i = 0; i < np; ++i) out[i] = -1;
+        for (std::size_t b = 0; b < lists.size(); ++b)
+            for (std::size_t i : lists[b]) out[i] = (long)b;
+        return 0;
+    }
 };
 
 template <typename T>
@@ -189,6 +211,28 @@ int ref_levels_f64(int dim, int poly_ord, int depth, int trans_sym, int kid, con
     case 1: return Stages<double, 1>().levels(poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, Wm, Wl);
     case 2: return Stages<double, 2>().levels(poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, Wm, Wl);
     case 3: return Stages<double, 3>().levels(poly_ord, depth, trans_sym, kid, prm, nt, t, ns, w, s, Wm, Wl);
+    }
+    return -1;
+}
+
+int ref_leaf_boxes_f64(int dim, int depth, long nt, const double* t, long ns, const double* s, long np,
+                       const double* pts, long* out)
+{
+    switch (dim) {
+    case 1: return Stages<double, 1>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
+    case 2: return Stages<double, 2>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
+    case 3: return Stages<double, 3>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
+    }
+    return -1;
+}
+
+int ref_leaf_boxes_f32(int dim, int depth, long nt, const float* t, long ns, const float* s, long np,
+                       const float* pts, long* out)
+{
+    switch (dim) {
+    case 1: return Stages<float, 1>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
+    case 2: return Stages<float, 2>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
+    case 3: return Stages<float, 3>().leaf_boxes(depth, nt, t, ns, s, np, pts, out);
     }
     return -1;
 }
