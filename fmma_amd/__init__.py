@@ -15,6 +15,10 @@ import numpy as np
 from . import _lib
 from ._lib import FMMAError, KERNELS, M2L_PRECISION  # noqa: F401
 
+# one HIP runtime for torch and this package, whichever is imported first
+# (fmma_amd.pyfmma links libfmma_hip.so directly)
+_lib.pin_hip_runtime()
+
 __all__ = ["Plan", "FMMAError", "KERNELS", "M2L_PRECISION", "version"]
 
 
@@ -94,6 +98,21 @@ class Plan:
         kid = KERNELS[name] if isinstance(name, str) else int(name)
         p = (ctypes.c_double * max(1, len(params)))(*params)
         _lib.check(self._L.fmma_plan_set_kernel(self._h, kid, p, len(params)))
+
+    def set_fn(self, fn, nargs=2):
+        """A Python kernel: fn(x, y) (nargs 2, the reference's set_fn2) or
+        fn(x - y) (nargs 1, set_fn1), written with arithmetic and numpy
+        functions over the coordinates x[0], x[1], ...; traced and compiled
+        for the GPU once (fmma_amd.jit), with its gradient."""
+        from . import jit
+        addr = jit.kernel_ops_address(fn, self.dim, self.dtype, nargs)
+        _lib.check(self._L.fmma_plan_set_kernel_ops(self._h, ctypes.c_void_p(addr)))
+
+    def set_fn1(self, fn):
+        self.set_fn(fn, 1)
+
+    def set_fn2(self, fn):
+        self.set_fn(fn, 2)
 
     def set_m2l_precision(self, prec):
         v = M2L_PRECISION[prec] if isinstance(prec, str) else int(prec)

@@ -55,11 +55,25 @@ _lib = None
 _hip = None
 
 
+def pin_hip_runtime():
+    """Load torch first when it is installed.  Its ROCm build brings its own
+    libamdhip64 (and HSA runtime); with torch's loaded first, this library
+    (and every Python kernel compiled by fmma_amd.jit) binds to that same
+    runtime, so device pointers and streams pass between them.  Loaded in
+    the other order, the process would hold two HIP runtimes and torch would
+    find no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library; raises if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    pin_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise ImportError("fmma_amd: %s is missing; build it with `python -m fmma_amd.build`" % LIB_PATH)
     L = ctypes.CDLL(LIB_PATH)

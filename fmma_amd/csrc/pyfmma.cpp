@@ -6,11 +6,14 @@
 // so every solve runs on the GPU.
 //
 // Shape checks and their RuntimeError messages follow the reference
-// (cpython/pyfmma.cpp:22-53).  Two deliberate differences: ans must already
+// (cpython/pyfmma.cpp:22-53).  One deliberate difference: ans must already
 // have the class's dtype and be C-contiguous (the reference would silently
-// fill a converted temporary otherwise), and set_fn1/set_fn2 reject Python
-// callables, which cannot run on the GPU -- use set_kernel(name, *params).
-// The GIL is released while the GPU works.
+// fill a converted temporary otherwise).  set_fn1(f) / set_fn2(f) take the
+// reference's Python callables, f(x - y) and f(x, y) over the point's
+// coordinates: fmma_amd.jit traces f once into an expression and compiles
+// it for the GPU (the reference calls f per pair on the CPU).  Built-in
+// kernels are also available by name with set_kernel(name, *params).  The
+// GIL is released while the GPU works.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -80,13 +83,21 @@ class PyFMMA : public fmma::FMMA<TYPE, DIM> {
         this->set_m2l_precision(v);
     }
 
-    void reject_callable(py::object)
+    // set_fn1 (nargs 1: f(x - y)) and set_fn2 (nargs 2: f(x, y))
+    // (cpython/pyfmma.cpp:150-151): the callable traced and compiled for the
+    // GPU by fmma_amd.jit, whose launcher table the plan copies.
+    void set_fn_py(py::object f, int nargs)
     {
-        throw std::runtime_error(
-            "a Python callable cannot run on the GPU: choose a built-in kernel with set_kernel(name, *params) "
-            "(inv_r, inv_r_reg, inv_diff, log_r, gauss, yukawa, sum_r2, sum2_r2) or pass a __host__ __device__ "
-            "functor to fmma::FMMA::set_fn from C++");
+        if (!PyCallable_Check(f.ptr())) throw std::runtime_error("set_fn needs a callable");
+        py::object jit = py::module_::import("fmma_amd.jit");
+        const uintptr_t addr = jit.attr("kernel_ops_address")(f, (int)DIM, sizeof(TYPE) == 8 ? "float64" : "float32",
+                                                              nargs)
+                                   .template cast<uintptr_t>();
+        const int rc = fmma_plan_set_kernel_ops(this->plan(), reinterpret_cast<const fmma_kernel_ops *>(addr));
+        if (rc != FMMA_OK) throw std::runtime_error(std::string("fmma: ") + fmma_last_error());
     }
+    void set_fn1_py(py::object f) { set_fn_py(f, 1); }
+    void set_fn2_py(py::object f) { set_fn_py(f, 2); }
 
   private:
     static void check_common(const py::buffer_info &bt, const py::buffer_info &bs, py::array &ans)
@@ -137,8 +148,8 @@ void make_class(py::module &m, const std::string &type_str)
         .def("solve", &C::pysolve_w, py::arg("target"), py::arg("source_weight"), py::arg("source"), py::arg("ans"))
         .def("solve", &C::pysolve, py::arg("target"), py::arg("source"), py::arg("ans"))
         .def("check_blas", &C::check_blas)
-        .def("set_fn1", &C::reject_callable)
-        .def("set_fn2", &C::reject_callable)
+        .def("set_fn1", &C::set_fn1_py, py::arg("fn"))
+        .def("set_fn2", &C::set_fn2_py, py::arg("fn"))
         .def("set_kernel", &C::set_kernel_py)
         .def("set_solver_type", &C::set_solver_type)
         .def("set_nrn_N", &C::set_nrn_N)

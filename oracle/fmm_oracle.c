@@ -99,6 +99,9 @@ double oracle_kernel(int id, const double *prm, int dim, const double *x, const 
         }
         return s2 / r2;
     }
+    case ORACLE_K_INV_R2:
+        for (int d = 0; d < dim; ++d) { double t = x[d] - y[d]; r2 += t * t; }
+        return 1.0 / r2;
     default:
         fprintf(stderr, "oracle: unknown kernel id %d\n", id);
         abort();
@@ -148,6 +151,9 @@ void oracle_kernel_grad(int id, const double *prm, int dim, const double *x, con
         for (int d = 0; d < dim; ++d) g[d] = 2.0 * (x[d] + y[d]) / r2 - 2.0 * s2 * t[d] / (r2 * r2);
         return;
     }
+    case ORACLE_K_INV_R2:
+        c = -2.0 / (r2 * r2);
+        break;
     default:
         fprintf(stderr, "oracle: unknown kernel id %d\n", id);
         abort();

@@ -17,7 +17,9 @@ enum {
     ORACLE_K_GAUSS = 4,     /* exp(-r^2/(2h^2)), prm[0]=h */
     ORACLE_K_YUKAWA = 5,    /* exp(-kappa r)/r, 0 at r=0, prm[0]=kappa */
     ORACLE_K_SUM_R2 = 6,    /* sum_d (x_d-y_d) / r^2 */
-    ORACLE_K_SUM2_R2 = 7    /* sum_d (x_d+y_d)^2 / r^2 (not translation invariant) */
+    ORACLE_K_SUM2_R2 = 7,   /* sum_d (x_d+y_d)^2 / r^2 (not translation invariant) */
+    ORACLE_K_INV_R2 = 8     /* 1/r^2, inf at r=0: the kernel of the reference's Python tutorials
+                               (tutorials/tutorial0{1..5}/*.py), checked there as a Python callable */
 };
 
 double oracle_kernel(int id, const double *prm, int dim, const double *x, const double *y);

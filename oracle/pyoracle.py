@@ -264,7 +264,7 @@ def fmm_grad(targets, weights, sources, poly_ord, depth=-1, trans_sym=0, kid=0, 
     return g
 
 
-SYMMETRY = {0: 1, 1: 1, 2: -1, 3: 1, 4: 1, 5: 1, 6: -1, 7: 1}
+SYMMETRY = {0: 1, 1: 1, 2: -1, 3: 1, 4: 1, 5: 1, 6: -1, 7: 1, 8: 1}
 
 
 def fmm_transpose(targets, cot, sources, poly_ord, depth=-1, trans_sym=0, kid=0, params=()):

@@ -65,6 +65,12 @@ std::function<T(const std::array<T, D>&, const std::array<T, D>&)> make_kernel(i
             for (std::size_t d = 0; d < D; ++d) { double t = x[d] - y[d]; r2 += t * t; s += t; }
             return (T)(s / r2);
         };
+    case 8:  // the Python tutorials' kernel (tutorials/tutorial0{1..5}/*.py)
+        return [](const A& x, const A& y) {
+            double r2 = 0.0;
+            for (std::size_t d = 0; d < D; ++d) { double t = x[d] - y[d]; r2 += t * t; }
+            return (T)(1.0 / r2);
+        };
     default:
         return [](const A& x, const A& y) {
             double r2 = 0.0, s2 = 0.0;

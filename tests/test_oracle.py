@@ -330,7 +330,8 @@ needs_ref = pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "
 
 @needs_ref
 @pytest.mark.parametrize("dim,n,p,depth,ts,kid", [(1, 3000, 6, -1, 0, 2), (2, 5000, 4, 5, 0, 3), (2, 4000, 5, 4, 1, 6),
-                                                  (3, 4000, 3, 4, 0, 1), (3, 6000, 2, 5, 1, 1), (3, 2000, 2, 3, 0, 7)])
+                                                  (3, 4000, 3, 4, 0, 1), (3, 6000, 2, 5, 1, 1), (3, 2000, 2, 3, 0, 7),
+                                                  (1, 2000, 3, 4, 0, 8), (2, 3000, 3, 4, 1, 8)])
 def test_oracle_matches_compiled_reference(oracle, dim, n, p, depth, ts, kid):
     t = synth.points(30, n, dim)
     s = synth.points(31, n, dim)

@@ -23,12 +23,14 @@ import pytest
 from fmma_amd import synth
 from tests.conftest import rel_l2
 
-KID = {"inv_r": 0, "inv_r_reg": 1, "inv_diff": 2, "log_r": 3, "gauss": 4, "yukawa": 5, "sum_r2": 6, "sum2_r2": 7}
+KID = {"inv_r": 0, "inv_r_reg": 1, "inv_diff": 2, "log_r": 3, "gauss": 4, "yukawa": 5, "sum_r2": 6, "sum2_r2": 7,
+       "inv_r2": 8}
 
 
 @pytest.mark.parametrize("name,dim,params", [("inv_r_reg", 3, ()), ("inv_r", 2, ()), ("inv_diff", 1, ()),
                                              ("log_r", 2, ()), ("gauss", 3, (0.3,)), ("yukawa", 3, (2.0,)),
-                                             ("sum_r2", 3, ()), ("sum2_r2", 2, ())])
+                                             ("sum_r2", 3, ()), ("sum2_r2", 2, ()), ("inv_r2", 2, ()),
+                                             ("inv_r2", 1, ())])
 def test_kernel_grad_matches_central_differences(oracle, name, dim, params):
     kid = KID[name]
     rng = np.random.default_rng(5)

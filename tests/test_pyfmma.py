@@ -1,7 +1,8 @@
-"""The pyfmma drop-in (fmma_amd.pyfmma): the reference module's classes and
-methods (cpython/pyfmma.cpp:140-168), its shape errors (cpython/pyfmma.cpp:22-53),
-and -- on the GPU -- its results against the oracle, used the way the
-reference's tutorials use it (tutorials/tutorial0*/tutorial0*.py)."""
+"""The pyfmma drop-in: `import pyfmma` as with the reference (setup.py:6),
+the reference module's classes and methods (cpython/pyfmma.cpp:140-168), its
+shape errors (cpython/pyfmma.cpp:22-53), Python kernels given to set_fn1 /
+set_fn2, and -- on the GPU -- its results against the oracle, used the way
+the reference's tutorials use it (tutorials/tutorial0*/tutorial0*.py)."""
 import numpy as np
 import pytest
 
@@ -10,8 +11,14 @@ from tests.conftest import rel_l2
 
 
 def _mod():
-    from fmma_amd import pyfmma
+    import pyfmma
     return pyfmma
+
+
+def test_top_level_module_is_the_extension():
+    import pyfmma
+    from fmma_amd import pyfmma as ext
+    assert pyfmma.fmmad3 is ext.fmmad3
 
 
 def test_module_surface():
@@ -40,10 +47,11 @@ def test_shape_errors_raise_runtime_error(bad, msg):
         f.solve(*bad())
 
 
-def test_python_callables_are_rejected():
+def test_untraceable_python_kernel_is_rejected():
+    """A callable that branches on a coordinate cannot be traced for the GPU."""
     f = _mod().fmmad1()
-    with pytest.raises(RuntimeError, match="cannot run on the GPU"):
-        f.set_fn2(lambda x, y: 1.0 / (x[0] - y[0]))
+    with pytest.raises(TypeError, match="branch"):
+        f.set_fn2(lambda x, y: 1.0 if x[0] > y[0] else -1.0)
 
 
 def test_unknown_kernel_name():
@@ -52,9 +60,9 @@ def test_unknown_kernel_name():
 
 
 @pytest.mark.gpu
-def test_tutorial05_style_fmm(oracle):
-    """tutorials/tutorial05/tutorial05.py with the 1/r^2-type kernel replaced by
-    a built-in (the tutorial's Python callable cannot run on the GPU)."""
+def test_builtin_kernel_tutorial05_layout(oracle):
+    """The layout of the reference's 2-D tutorial (points on the diagonal,
+    weights i/N, Depth 3, order 3) with a built-in kernel by name."""
     pyfmma = _mod()
     f = pyfmma.fmmad2()
     f.set_kernel("sum_r2")
