@@ -227,6 +227,14 @@ __device__ __forceinline__ void target_coords(const M2LTask &t, int tid, int (&t
 // LDS a pipelined M2L workgroup may take for two operand tiles.
 constexpr int kM2LDoubleLds = 160 * 1024 - 4096;
 
+// Row padding (elements) of an LDS operand tile [column][k]: one element
+// for FP64 and FP32 (an odd row length spreads the 16 columns of an MFMA
+// B-operand read over the banks; KC + 2 measured 5x the FP64 bank
+// conflicts of KC + 1, and KC + 4 did not change FP32's), 8 for BF16 (16-byte
+// rows of 16 columns on distinct banks).
+template <int ESIZE>
+constexpr int kLdsPad = ESIZE == 2 ? 8 : 1;
+
 // Operand rows staged per LDS pass of m2l_mfma4: 64 boxes x (256 + 1) FP64
 // values = 132 KB, inside the 160 KB of a CU for every P <= 512.
 constexpr int kM2LChunk = 256;
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(256) void m2l_mfma4(M2LKArgs a)
     const int rb = blockIdx.y * 4 + wave;
     const bool active = rb * 16 < a.MP;
     const int N = 1 << task.level, P = a.P, KP = a.KP;
-    const int KC = KP < kM2LChunk ? KP : kM2LChunk, ldb = KC + 1;
+    const int KC = KP < kM2LChunk ? KP : kM2LChunk, ldb = KC + kLdsPad<sizeof(CT)>;
     int tc[D];
     bool tvalid;
     target_coords<D>(task, tid, tc, tvalid);
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     using V4 = typename std::conditional<MODE == 0, d4_t, f4_t>::type;
     using ST = typename std::conditional<MODE == 2, __bf16, CT>::type;
     using AT = typename std::conditional<MODE == 2, bf8_t, CT>::type;
-    constexpr int LDB = MODE == 2 ? KC + 8 : KC + 1;
+    constexpr int LDB = KC + kLdsPad<sizeof(ST)>;
     constexpr int NPART = MODE == 2 ? 2 : 1;
     constexpr int VEC = 16 / (int)sizeof(ST);  // elements per 16-byte load
     static_assert(EPT % VEC == 0, "gather chunks must be whole 16-byte vectors");
@@ -719,7 +727,8 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
             if constexpr (std::is_same<T, double>::value) {
                 done = g.pipelined && launch_pipe<T, D, 0>(a, g.tb, grid, st);
                 if (!done) {
-                    const size_t lds = (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + 1) * sizeof(double);
+                    const size_t lds =
+                        (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + kLdsPad<8>) * sizeof(double);
                     hipLaunchKernelGGL((m2l_mfma4<T, D, 0>), grid, dim3(256), lds, st, a);
                 }
             } else {
@@ -728,7 +737,8 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
         } else if (mode == 1) {
             done = g.pipelined && launch_pipe<T, D, 1>(a, g.tb, grid, st);
             if (!done) {
-                const size_t lds = (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + 1) * sizeof(float);
+                const size_t lds =
+                    (size_t)64 * ((a.KP < kM2LChunk ? a.KP : kM2LChunk) + kLdsPad<4>) * sizeof(float);
                 hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
             }
         } else {

@@ -185,6 +185,7 @@ struct Plan {
     DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks, d_op_levels, d_op_prob, d_pairs;
     int n_pairs = 0;
     DBuf h_t, h_s, h_w, h_ans;
+    DBuf rec;  // packed point records (coordinates, weight) for the gather after the sort
 
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
     std::unique_ptr<BatchPlan> batch;
@@ -458,13 +459,17 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
         (rc = pos.ensure((size_t)n * D * sizeof(T))) || (rc = start.ensure(((size_t)nbox + 1) * 4)))
         return rc;
     if (with_w && (rc = wbuf->ensure((size_t)n * sizeof(T)))) return rc;
+    if ((rc = pl.rec.ensure((size_t)kRecStride<D> * n * sizeof(T)))) return rc;
     const Geom<T> *g = pl.geom.as<Geom<T>>();
-    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream))) return rc;
+    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream, with_w ? w : nullptr,
+                              pl.rec.as<T>())))
+        return rc;
     SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
                    hist.as<unsigned>()};
     if ((rc = radix_sort(sb, n, bits, pl.stream))) return rc;
     if ((rc = box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream))) return rc;
-    return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, pl.stream);
+    return gather_records<T, D>(pl.rec.as<T>(), vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr,
+                                pl.stream);
 }
 
 // The M2L tile autotuner for one problem: per level, time every pipelined

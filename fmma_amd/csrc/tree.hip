@@ -138,12 +138,26 @@ int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_si
 
 // ---------------------------------------------------------------- leaf keys
 
+// With rec != NULL the kernel also packs each point into one aligned record
+// (its D coordinates and its weight, kRecStride<D> values), so that the
+// gather after the sort reads one record per point instead of a coordinate
+// triple and a weight from two random places.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
-                                                   int grid_n, unsigned *keys)
+                                                   int grid_n, unsigned *keys, const T *w, T *rec)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    if (rec) {
+        T r[kRecStride<D>];
+#pragma unroll
+        for (int d = 0; d < D; ++d) r[d] = pts[i * D + d];
+        r[D] = w ? w[i] : T(1);
+#pragma unroll
+        for (int d = D + 1; d < kRecStride<D>; ++d) r[d] = T(0);
+#pragma unroll
+        for (int d = 0; d < kRecStride<D>; ++d) rec[i * kRecStride<D> + d] = r[d];
+    }
     const int NL = grid_n > 0 ? grid_n : 1 << leaf_level;
     const T len = g->len_leaf;
     int b[D];
@@ -160,11 +174,12 @@ __global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const G
 }
 
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st)
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st,
+              const T *w, T *rec)
 {
     if (n <= 0) return FMMA_OK;
     hipLaunchKernelGGL((keys_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, n, g,
-                       leaf_level, grid_n, keys);
+                       leaf_level, grid_n, keys, w, rec);
     FMMA_LAUNCHED("leaf_keys");
     return FMMA_OK;
 }
@@ -407,6 +422,32 @@ __global__ __launch_bounds__(256) void gather_kernel(const T *pts, const T *w, c
     if (ws) ws[i] = w ? w[p] : T(1);
 }
 
+// The same from the packed records of keys_kernel: one aligned record per
+// point (16 or 32 bytes), read with vector loads.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void gather_records_kernel(const T *rec, const unsigned *perm, long n, T *pos, T *ws)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned p = perm[i];
+    T r[kRecStride<D>];
+#pragma unroll
+    for (int d = 0; d < kRecStride<D>; ++d) r[d] = rec[(long)p * kRecStride<D> + d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pos[d * n + i] = r[d];
+    if (ws) ws[i] = r[D];
+}
+
+template <typename T, int D>
+int gather_records(const T *rec, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    hipLaunchKernelGGL((gather_records_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rec, perm,
+                       n, pos, ws);
+    FMMA_LAUNCHED("gather_records");
+    return FMMA_OK;
+}
+
 template <typename T, int D>
 int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st)
 {
@@ -419,8 +460,10 @@ int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws
 
 #define FMMA_TREE_INST(T, D)                                                                                    \
     template int bbox<T, D>(const T *, long, const T *, long, T *, Geom<T> *, int, hipStream_t);                \
-    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t);          \
-    template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);
+    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t, const T *,  \
+                                 T *);                                                                         \
+    template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);              \
+    template int gather_records<T, D>(const T *, const unsigned *, long, T *, T *, hipStream_t);
 FMMA_TREE_INST(double, 1)
 FMMA_TREE_INST(double, 2)
 FMMA_TREE_INST(double, 3)
