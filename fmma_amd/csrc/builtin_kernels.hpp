@@ -40,6 +40,23 @@ __device__ __forceinline__ float rsqrt_refined(float r2)
     return __builtin_fmaf(y * e, 0.5f, y);
 }
 
+// 2/sqrt(r2): the refined value of rsqrt_refined times 2, as
+// y0 (3 - r2 y0^2), one instruction fewer; the pair loops divide the
+// weights by the factor 2 (pair_scale).
+__device__ __forceinline__ double rsqrt2_refined(double r2)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double h = r2 * y;
+    return y * __builtin_fma(-h, y, 3.0);
+}
+
+__device__ __forceinline__ float rsqrt2_refined(float r2)
+{
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float h = r2 * y;
+    return y * __builtin_fmaf(-h, y, 3.0f);
+}
+
 template <typename T, int D>
 __device__ __forceinline__ T dist2(const T *x, const T *y)
 {
@@ -90,6 +107,7 @@ struct KInvR {  // reference default: 1/|x-y|, inf at r = 0 (include/fmma/fmma.h
 struct KInvRReg {  // 1/|x-y|, 0 at r = 0
     static constexpr bool translation_invariant = true;
     static constexpr int symmetry = 1;
+    static constexpr double pair_scale = 2.0;  // the pair loops evaluate 2/r
     double p0, p1;
     template <typename T, int D>
     __device__ __forceinline__ T operator()(const T *x, const T *y) const
@@ -97,6 +115,18 @@ struct KInvRReg {  // 1/|x-y|, 0 at r = 0
         const T r2 = dist2<T, D>(x, y);
         const T v = rsqrt_refined(r2);
         return r2 == T(0) ? T(0) : v;
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ T near_scaled(const T *x, const T *y) const
+    {
+        const T r2 = dist2<T, D>(x, y);
+        const T v = rsqrt2_refined(r2);
+        return r2 == T(0) ? T(0) : v;
+    }
+    template <typename T, int D>
+    __device__ __forceinline__ T far_scaled(const T *x, const T *y) const
+    {
+        return rsqrt2_refined(dist2<T, D>(x, y));
     }
     template <typename T, int D>
     __device__ __forceinline__ T far(const T *x, const T *y) const

@@ -626,10 +626,16 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     }
 }
 
-// The contraction chunk of a padded length: all of it up to 128; beyond,
-// 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a multiple of
-// 128), so that a step's MFMA work still covers the next step's loads.
-constexpr int m2l_chunk(int KP, int mode) { return KP <= 128 ? KP : mode == 2 ? 128 : 32; }
+// The contraction chunk of a padded length: all of it below 128 (and at 128
+// for FP32 and BF16x3); FP64 at 128 in chunks of 64 (the smaller operand
+// tiles and fragments raise occupancy: 6.0 -> 5.0 ms at 5 nodes, N = 4M);
+// beyond 128, 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a
+// multiple of 128), so that a step's MFMA work still covers the next step's
+// loads.
+constexpr int m2l_chunk(int KP, int mode)
+{
+    return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 64 : 128) : 32;
+}
 
 // Tile widths with a pipelined kernel for (MODE, KP, TB); the others use
 // m2l_mfma4 / m2l_bf16x3.

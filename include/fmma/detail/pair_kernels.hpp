@@ -50,13 +50,43 @@ struct kernel_symmetry<K, std::void_t<decltype(K::symmetry)>> {
     static constexpr int value = K::symmetry;
 };
 
+// A functor may evaluate c*f instead of f in the pair loops (P2P, M2P) when
+// that is cheaper, declaring `static constexpr double pair_scale = c` (a power
+// of two) and near_scaled() / far_scaled(); the loops then carry every weight
+// divided by c, which is exact, so sum w f = sum (w / c)(c f).  1/r uses it to
+// fold the Newton step's factor 1/2 into the weights.
+template <class K, typename = void>
+struct pair_scale {
+    static constexpr double value = 1.0;
+};
+template <class K>
+struct pair_scale<K, std::void_t<decltype(K::pair_scale)>> {
+    static constexpr double value = K::pair_scale;
+};
+
+// The weight as the pair loops carry it.
+template <class K, typename T>
+__device__ __forceinline__ T pair_weight(T w)
+{
+    if constexpr (pair_scale<K>::value != 1.0)
+        return w * T(1.0 / pair_scale<K>::value);
+    else
+        return w;
+}
+
 template <bool FAR, typename T, int D, class K>
 __device__ __forceinline__ T pair_eval(const K &k, const T *x, const T *y)
 {
-    if constexpr (FAR && has_far<K, T, D>::value)
+    if constexpr (pair_scale<K>::value != 1.0) {
+        if constexpr (FAR)
+            return k.template far_scaled<T, D>(x, y);
+        else
+            return k.template near_scaled<T, D>(x, y);
+    } else if constexpr (FAR && has_far<K, T, D>::value) {
         return k.template far<T, D>(x, y);
-    else
+    } else {
         return k.template operator()<T, D>(x, y);
+    }
 }
 
 // A functor may provide grad(x, y, g): g[d] = d/dx_d f(x, y) (the near
@@ -271,7 +301,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
                 T *r = src + e * RW;
 #pragma unroll
                 for (int d = 0; d < D; ++d) r[d] = sp[d * ns + j];
-                r[D] = sw[j];
+                r[D] = pair_weight<K>(sw[j]);
             }
             __syncthreads();
             const unsigned ns_blk = nself > base ? nself - base : 0u;
@@ -516,7 +546,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
             const unsigned ic = i < p1 ? i : p1 - 1;
 #pragma unroll
             for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? pos[d * n + ic] : T(0);
-            wt[k] = k < tpl && i < p1 ? pw[i] : T(0);  // padding targets add nothing to B
+            wt[k] = k < tpl && i < p1 ? pair_weight<K>(pw[i]) : T(0);  // padding targets add nothing to B
         }
         for (unsigned base = 0; base < total; base += kP2PBlock) {
             const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
@@ -533,7 +563,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2
                 T *r = src + e * RW;
 #pragma unroll
                 for (int d = 0; d < D; ++d) r[d] = pos[d * n + j];
-                r[D] = e < cnt ? pw[j] : T(0);
+                r[D] = e < cnt ? pair_weight<K>(pw[j]) : T(0);
             }
             __syncthreads();
             if (small)
@@ -1251,7 +1281,7 @@ __global__ __launch_bounds__(64) void m2p_kernel(fmma_m2p_args a, K kern)
                             kk /= nn;
                             r[d] = (T)(hc[dg] * ln + (double)ent_rel[ent * D + d]);
                         }
-                        r[D] = Wm[ent_off[ent] + k];
+                        r[D] = pair_weight<K>(Wm[ent_off[ent] + k]);
                     }
                     __syncthreads();
                     p2p_block_far<T, D>(tpl, src, cnt, q, nslice, x, acc, kern);
