@@ -261,31 +261,44 @@ __device__ __forceinline__ void cheb_weights_ct(const T *coef, T r, T scale, T (
 
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
-// P2M: one wave per source box, lane = coefficient (KPL per lane).
+// P2M: one wave per source box.  The box's points are staged 64 at a time,
+// each lane computing one point's Chebyshev weights S_d (D x NN, the first
+// scaled by the point's weight) into LDS.  Then the multipole
+// Wm[a0..a_{D-1}] = sum_p S_0[p][a0] ... S_{D-1}[p][a_{D-1}] is accumulated as
+// a sum of outer products: lane (g, c) holds the trailing index tuple c of
+// the last D - 1 dimensions (NN^(D-1) of them) and all NN coefficients along
+// dimension 0 in registers, and takes every G-th point (g = lane group, G =
+// 64 / NN^(D-1) groups); the groups are added at the end.  Per point and lane
+// that is one product of D - 1 weights and NN multiply-adds, for NN
+// coefficients.
 template <typename T, int D, int NN>
 __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
 {
-    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64;
+    constexpr int P = ipow_c(NN, D);
+    constexpr int NT = ipow_c(NN, D - 1);  // trailing index tuples
+    constexpr int G = NT <= 64 ? 64 / NT : 1;
+    static_assert(NT <= 64, "at most 64 trailing index tuples (8 nodes in 3-D)");
     __shared__ T sc[NN * NN];
-    __shared__ T sv[64][D * NN];
+    __shared__ T sv[64][D * NN + 1];
+    __shared__ T red[G > 1 ? G - 1 : 1][NT * NN];
     const unsigned box = blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < NN * NN; i += 64) sc[i] = a.coef[i];
     const unsigned s0 = a.start[box], s1 = a.start[box + 1];
-    int dig[KPL][D];
+    const int g = lane / NT, c = lane - g * NT;
+    const bool act = g < G;
+    int dig[D > 1 ? D - 1 : 1];
+    {
+        int k = act ? c : 0;
 #pragma unroll
-    for (int r = 0; r < KPL; ++r) {
-        int k = lane + 64 * r;
-        k = k < P ? k : 0;
-#pragma unroll
-        for (int d = D - 1; d >= 0; --d) {
-            dig[r][d] = k % NN;
+        for (int d = D - 1; d >= 1; --d) {
+            dig[d - 1] = k % NN;
             k /= NN;
         }
     }
-    T acc[KPL];
+    T acc[NN];
 #pragma unroll
-    for (int r = 0; r < KPL; ++r) acc[r] = T(0);
+    for (int k = 0; k < NN; ++k) acc[k] = T(0);
     const Geom<T> *gm;
     const int lv = box_geom<T>(a, box, gm), NL = a.grid_n > 0 ? a.grid_n : 1 << lv;
     const T len = gm->len_leaf;
@@ -306,22 +319,31 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
         }
         __syncthreads();
         const int cnt = (int)(s1 - base < 64u ? s1 - base : 64u);
-        for (int p = 0; p < cnt; ++p) {
+        if (act) {
+            for (int p = g; p < cnt; p += G) {
+                T t = T(1);
 #pragma unroll
-            for (int r = 0; r < KPL; ++r) {
-                T v = sv[p][dig[r][0]];
+                for (int d = 1; d < D; ++d) t *= sv[p][d * NN + dig[d - 1]];
 #pragma unroll
-                for (int d = 1; d < D; ++d) v *= sv[p][d * NN + dig[r][d]];
-                acc[r] += v;
+                for (int k = 0; k < NN; ++k) acc[k] += sv[p][k] * t;
             }
         }
         __syncthreads();
     }
+    // groups 1..G-1 hand their sums to group 0
+    if constexpr (G > 1) {
+        if (act && g > 0)
 #pragma unroll
-    for (int r = 0; r < KPL; ++r) {
-        const int k = lane + 64 * r;
-        if (k < P) a.Wm[(size_t)box * P + k] = acc[r];
+            for (int k = 0; k < NN; ++k) red[g - 1][k * NT + c] = acc[k];
+        __syncthreads();
+        if (g == 0)
+#pragma unroll
+            for (int k = 0; k < NN; ++k)
+                for (int h = 1; h < G; ++h) acc[k] += red[h - 1][k * NT + c];
     }
+    if (g == 0)
+#pragma unroll
+        for (int k = 0; k < NN; ++k) a.Wm[(size_t)box * P + k * NT + c] = acc[k];
 }
 
 // out[..k_d..] = sum_c M[k_d][c] in[..c..] along dimension d of an NN^D
@@ -474,24 +496,33 @@ __global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
         leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
         cheb_weights_ct<T, NN>(sc, rel, T(1), S[d]);
     }
+    // the contraction one dimension at a time, innermost (last) first:
+    // independent short sums instead of one chain over all P coefficients
     T acc = T(0);
     if constexpr (D == 1) {
 #pragma unroll
         for (int a0 = 0; a0 < NN; ++a0) acc += W[a0] * S[0][a0];
     } else if constexpr (D == 2) {
 #pragma unroll
-        for (int a0 = 0; a0 < NN; ++a0)
+        for (int a0 = 0; a0 < NN; ++a0) {
+            T t = T(0);
 #pragma unroll
-            for (int a1 = 0; a1 < NN; ++a1) acc += W[a0 * NN + a1] * (S[0][a0] * S[1][a1]);
+            for (int a1 = 0; a1 < NN; ++a1) t += W[a0 * NN + a1] * S[1][a1];
+            acc += S[0][a0] * t;
+        }
     } else {
 #pragma unroll
-        for (int a0 = 0; a0 < NN; ++a0)
+        for (int a0 = 0; a0 < NN; ++a0) {
+            T u = T(0);
 #pragma unroll
             for (int a1 = 0; a1 < NN; ++a1) {
-                const T s01 = S[0][a0] * S[1][a1];
+                T t = T(0);
 #pragma unroll
-                for (int a2 = 0; a2 < NN; ++a2) acc += W[(a0 * NN + a1) * NN + a2] * (s01 * S[2][a2]);
+                for (int a2 = 0; a2 < NN; ++a2) t += W[(a0 * NN + a1) * NN + a2] * S[2][a2];
+                u += S[1][a1] * t;
             }
+            acc += S[0][a0] * u;
+        }
     }
     a.out[i] = acc;
 }
