@@ -421,8 +421,10 @@ __global__ __launch_bounds__(256) void m2l_bf16x3(M2LKArgs a)
 // are prefetched the same way.  Accumulators stay in registers across all
 // steps: FP64 in MODE 0; in MODES 1-2 the MFMA's FP32 accumulator sums one
 // offset and is then added into FP64 registers, so rounding in FP32 covers
-// KP products, not the whole interaction list.
-template <typename TW, int D, int MODE, int KP, int TB, int KC>
+// KP products, not the whole interaction list.  Each wave owns RB row blocks
+// of 16 rows: every B fragment it reads from LDS feeds RB MFMAs, and a
+// workgroup covers 64 RB rows, so fewer workgroups gather each partner tile.
+template <typename TW, int D, int MODE, int KP, int TB, int KC, int RB>
 __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 {
     static_assert(KP % KC == 0, "the chunk divides the padded length");
@@ -451,8 +453,11 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     const unsigned tix = xcd * nt8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
     const M2LTask task = a.tasks[tix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int rb = blockIdx.y * 4 + wave;
-    const bool active = rb * 16 < a.MP;
+    const int rb0 = (blockIdx.y * 4 + wave) * RB;
+    bool act[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) act[r] = (rb0 + r) * 16 < a.MP;
+    const bool active = act[0];
     const int N = 1 << task.level, P = a.P, nrb = a.MP / 16;
     const int col = tid / TPC, part = tid % TPC;
 
@@ -490,38 +495,45 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
             for (int e = 0; e < EPT / VEC; ++e) g_lo[e] = ok ? sl[e] : uint4{0, 0, 0, 0};
         }
     };
-    AT afr[NKC];
-    [[maybe_unused]] AT afr_lo[MODE == 2 ? NKC : 1];
+    AT afr[RB][NKC];
+    [[maybe_unused]] AT afr_lo[RB][MODE == 2 ? NKC : 1];
     auto load_a = [&](int j) {
-        if (!active) return;
         const int i = NCH == 1 ? j : j / NCH, c = NCH == 1 ? 0 : j - i * NCH;
         const size_t f = (size_t)cls_f[i];
-        const size_t base = ((f * nrb + rb) * NKS + (size_t)c * NKC) * 64;
-        if constexpr (MODE == 2) {
-            const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
-            const bf8_t *Kl =
-                reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
 #pragma unroll
-            for (int s = 0; s < NKC; ++s) {
-                afr[s] = Kh[base + s * 64 + lane];
-                afr_lo[s] = Kl[base + s * 64 + lane];
+        for (int r = 0; r < RB; ++r) {
+            if (!act[r]) continue;
+            const size_t base = ((f * nrb + rb0 + r) * NKS + (size_t)c * NKC) * 64;
+            if constexpr (MODE == 2) {
+                const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
+                const bf8_t *Kl =
+                    reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
+#pragma unroll
+                for (int s = 0; s < NKC; ++s) {
+                    afr[r][s] = Kh[base + s * 64 + lane];
+                    afr_lo[r][s] = Kl[base + s * 64 + lane];
+                }
+            } else {
+                const CT *Kf = static_cast<const CT *>(a.Kf) + task.k_off;
+#pragma unroll
+                for (int s = 0; s < NKC; ++s) afr[r][s] = Kf[base + s * 64 + lane];
             }
-        } else {
-            const CT *Kf = static_cast<const CT *>(a.Kf) + task.k_off;
-#pragma unroll
-            for (int s = 0; s < NKC; ++s) afr[s] = Kf[base + s * 64 + lane];
         }
     };
 
-    V4 acc[NCB];
+    V4 acc[RB][NCB];
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) acc[cb] = V4{0, 0, 0, 0};
-    [[maybe_unused]] double accd[MODE == 0 ? 1 : NCB][4];
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) acc[r][cb] = V4{0, 0, 0, 0};
+    [[maybe_unused]] double accd[MODE == 0 ? 1 : RB][MODE == 0 ? 1 : NCB][4];
     if constexpr (MODE != 0) {
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb)
+        for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) accd[cb][q] = 0.0;
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) accd[r][cb][q] = 0.0;
     }
 
     const int n = a.n_cls * NCH;
@@ -551,13 +563,15 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
             store_tile(Bs);
             __syncthreads();
         }
-        AT acur[NKC];
-        [[maybe_unused]] AT acur_lo[MODE == 2 ? NKC : 1];
+        AT acur[RB][NKC];
+        [[maybe_unused]] AT acur_lo[RB][MODE == 2 ? NKC : 1];
 #pragma unroll
-        for (int s = 0; s < NKC; ++s) {
-            acur[s] = afr[s];
-            if constexpr (MODE == 2) acur_lo[s] = afr_lo[s];
-        }
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int s = 0; s < NKC; ++s) {
+                acur[r][s] = afr[r][s];
+                if constexpr (MODE == 2) acur_lo[r][s] = afr_lo[r][s];
+            }
         if (i + 1 < n) {
             gather(i + 1);
             load_a(i + 1);
@@ -572,9 +586,14 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                         const int off = (16 * cb + (lane & 15)) * LDB + kk;
                         const bf8_t bh = *reinterpret_cast<const bf8_t *>(cur + off);
                         const bf8_t bl = *reinterpret_cast<const bf8_t *>(cur + TB * LDB + off);
-                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[s], bh, acc[cb], 0, 0, 0);
-                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bl, acc[cb], 0, 0, 0);
-                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[s], bh, acc[cb], 0, 0, 0);
+#pragma unroll
+                        for (int r = 0; r < RB; ++r) {
+                            if (!act[r]) continue;
+                            acc[r][cb] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur_lo[r][s], bh, acc[r][cb], 0, 0, 0);
+                            acc[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[r][s], bl, acc[r][cb], 0, 0, 0);
+                            acc[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[r][s], bh, acc[r][cb], 0, 0, 0);
+                        }
                     }
                 }
             } else {
@@ -584,21 +603,27 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
 #pragma unroll
                     for (int cb = 0; cb < NCB; ++cb) {
                         const CT bv = cur[(16 * cb + (lane & 15)) * LDB + kk];
-                        if constexpr (MODE == 0)
-                            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[s], bv, acc[cb], 0, 0, 0);
-                        else
-                            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[s], bv, acc[cb], 0, 0, 0);
+#pragma unroll
+                        for (int r = 0; r < RB; ++r) {
+                            if (!act[r]) continue;
+                            if constexpr (MODE == 0)
+                                acc[r][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[r][s], bv, acc[r][cb], 0, 0, 0);
+                            else
+                                acc[r][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[r][s], bv, acc[r][cb], 0, 0, 0);
+                        }
                     }
                 }
             }
             if constexpr (MODE != 0) {
                 if (NCH == 1 || (i + 1) % NCH == 0) {  // one offset summed: into FP64
 #pragma unroll
-                    for (int cb = 0; cb < NCB; ++cb) {
+                    for (int r = 0; r < RB; ++r)
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) accd[cb][q] += (double)acc[cb][q];
-                        acc[cb] = V4{0, 0, 0, 0};
-                    }
+                        for (int cb = 0; cb < NCB; ++cb) {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) accd[r][cb][q] += (double)acc[r][cb][q];
+                            acc[r][cb] = V4{0, 0, 0, 0};
+                        }
                 }
             }
         }
@@ -615,12 +640,16 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         if (cc >= task.np) continue;
         const size_t box = ((size_t)(task.p0 + cc) << D) | (unsigned)task.cls;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
-            if constexpr (MODE == 0) {
-                if (row < P) Wl[box * P + row] = (TW)acc[cb][q];
-            } else {
-                if (row < P) Wl[box * P + row] = (TW)accd[cb][q];
+        for (int r = 0; r < RB; ++r) {
+            const int rb = rb0 + r;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = MODE == 0 ? 16 * rb + (lane >> 4) + 4 * q : 16 * rb + 4 * (lane >> 4) + q;
+                if constexpr (MODE == 0) {
+                    if (row < P) Wl[box * P + row] = (TW)acc[r][cb][q];
+                } else {
+                    if (row < P) Wl[box * P + row] = (TW)accd[r][cb][q];
+                }
             }
         }
     }
@@ -637,6 +666,11 @@ constexpr int m2l_chunk(int KP, int mode)
     return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 64 : 128) : 32;
 }
 
+// Row blocks of 16 per wave: 2 for BF16x3 at a padded length of 128 (one
+// workgroup covers all MP <= 128 rows; measured 1.15 -> 1.10 ms at 5 nodes,
+// N = 4M), else 1 (FP32 at 2 measured 2.7 -> 3.8 ms, FP64 unchanged).
+constexpr int m2l_row_blocks(int KP, int mode) { return KP == 128 && mode == 2 ? 2 : 1; }
+
 // Tile widths with a pipelined kernel for (MODE, KP, TB); the others use
 // m2l_mfma4 / m2l_bf16x3.
 template <typename T, int D, int MODE, int KP, int TB>
@@ -644,11 +678,13 @@ static bool launch_pipe_tb(const M2LKArgs &a, dim3 grid, hipStream_t st)
 {
     constexpr int VEC = MODE == 2 ? 8 : MODE == 1 ? 4 : 2;
     constexpr int KC = m2l_chunk(KP, MODE);
+    constexpr int RB = m2l_row_blocks(KP, MODE);
     if constexpr ((KC / (256 / TB)) % VEC != 0 || (MODE == 2 && KP % 32 != 0) || (MODE == 0 && TB > 128) ||
                   (MODE == 2 && KP > 128 && TB > 128)) {
         return false;
     } else {
-        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC>), grid, dim3(256), 0, st, a);
+        const dim3 g2(grid.x, (unsigned)((a.MP / 16 + 4 * RB - 1) / (4 * RB)));
+        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC, RB>), g2, dim3(256), 0, st, a);
         return true;
     }
 }
