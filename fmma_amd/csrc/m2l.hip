@@ -79,17 +79,72 @@ void OffsetTables::build(int dim)
     }
 }
 
+// The contraction chunk of a padded length: all of it below 128 (and at 128
+// for FP32 and BF16x3); FP64 at 128 in chunks of 64 (the smaller operand
+// tiles and fragments raise occupancy: 6.0 -> 5.0 ms at 5 nodes, N = 4M);
+// beyond 128, 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a
+// multiple of 128), so that a step's MFMA work still covers the next step's
+// loads.
+constexpr int m2l_chunk(int KP, int mode)
+{
+    return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 64 : 128) : 32;
+}
+
+// Row blocks of 16 per wave: 2 for BF16x3 at a padded length of 128 (one
+// workgroup covers all MP <= 128 rows; measured 1.15 -> 1.10 ms at 5 nodes,
+// N = 4M), else 1 (FP32 at 2 measured 2.7 -> 3.8 ms, FP64 unchanged).
+constexpr int m2l_row_blocks(int KP, int mode) { return KP == 128 && mode == 2 ? 2 : 1; }
+
+// Reorders the tasks [first, first + count) of one pipelined launch, made of
+// runs of unequal cost (levels; a split level's tasks are shorter), so that
+// the contiguous range m2l_pipe gives each XCD (below) holds an equal share
+// of every run, each share contiguous in the original order (neighbouring
+// tiles on one XCD share partners in its L2).  Run remainders are dealt to
+// the XCDs cyclically, which gives XCD x exactly the nt8 + (x < rem) tasks
+// the kernel's mapping assigns it.
+static void xcd_balance(std::vector<M2LTask> &t, int first, int count, const std::vector<int> &runs)
+{
+    std::vector<M2LTask> per[8];
+    int ptr = 0;
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const int a = runs[r], b = r + 1 < runs.size() ? runs[r + 1] : first + count;
+        const int n = b - a, base = n / 8, rem = n % 8;
+        int pos = a;
+        for (int x = 0; x < 8; ++x) {
+            const int size = base + ((x - ptr + 8) % 8 < rem ? 1 : 0);
+            per[x].insert(per[x].end(), t.begin() + pos, t.begin() + pos + size);
+            pos += size;
+        }
+        ptr = (ptr + rem) % 8;
+    }
+    int o = first;
+    for (int x = 0; x < 8; ++x)
+        for (const M2LTask &k : per[x]) t[o++] = k;
+}
+
 // Tasks level by level, deepest first.  A level's tile width is the widest
 // of 256/128/64 boxes that still gives >= 512 workgroups (two per CU), when a
-// pipelined kernel exists for it; tasks of equal width form one launch.
+// pipelined kernel exists for it; tasks of equal width form one launch.  A
+// split level's parts of one tile are consecutive tasks (the same XCD).
 std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
                                     const long *w_off, const long *c_off, const long *k_off, const int *tb_of_level,
-                                    int only_level)
+                                    int only_level, std::vector<M2LSplit> *splits, long split_target)
 {
     std::vector<M2LTask> t;
     groups.clear();
+    if (splits) splits->clear();
     int MP, KP;
     m2l_shape(P, mode, MP, KP);
+    int n_cls = 1, n3 = 1;
+    for (int d = 0; d < D; ++d) {
+        n_cls *= 6;
+        n3 *= 3;
+    }
+    n_cls -= n3;  // interaction offsets per class
+    const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
+    const long gy = (MP / 16 + 4 * rb - 1) / (4 * rb);  // workgroups per task (row chunks)
+    long part_off = 0;
+    std::vector<std::vector<int>> runs;  // per group: the first task of each level
     for (int l = leaf_level; l >= 2; --l) {
         if (only_level >= 0 && l != only_level) continue;
         const unsigned npar = 1u << (D * (l - 1));
@@ -102,16 +157,49 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         }
         if (tb_of_level && tb_of_level[l] > 0) tb = tb_of_level[l];
         const bool piped = m2l_pipe_supported(D, KP, mode, tb);
-        if (groups.empty() || groups.back().tb != tb || groups.back().pipelined != piped)
+        if (groups.empty() || groups.back().tb != tb || groups.back().pipelined != piped) {
             groups.push_back(M2LGroup{tb, (int)t.size(), 0, piped});
+            runs.emplace_back();
+        }
+        runs.back().push_back((int)t.size());
+        int parts = 1;
+        if (splits && split_target > 0 && piped) {
+            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * gy;
+            long p = split_target / wg;
+            if (p > 8) p = 8;
+            if (p > n_cls / 16) p = n_cls / 16;
+            parts = p >= 2 ? (int)p : 1;
+        }
+        const long count = (long)P << (D * l);
         for (unsigned p0 = 0; p0 < npar; p0 += tb)
             for (int c = 0; c < (1 << D); ++c) {
                 const unsigned np = npar - p0 < (unsigned)tb ? npar - p0 : (unsigned)tb;
-                t.push_back(M2LTask{l, c, p0, (int)np, w_off[l], c_off[l], k_off[l]});
-                ++groups.back().count;
+                for (int q = 0; q < parts; ++q) {
+                    M2LTask k{l, c, p0, (int)np, w_off[l], c_off[l], k_off[l]};
+                    if (parts > 1) {
+                        k.i0 = q * n_cls / parts;
+                        k.i1 = (q + 1) * n_cls / parts;
+                        k.o_off = part_off + q * count;
+                    }
+                    t.push_back(k);
+                    ++groups.back().count;
+                }
             }
+        if (parts > 1) {
+            splits->push_back(M2LSplit{l, parts, w_off[l], count, part_off});
+            part_off += parts * count;
+        }
     }
+    for (size_t g = 0; g < groups.size(); ++g)
+        if (groups[g].pipelined && runs[g].size() > 1) xcd_balance(t, groups[g].first, groups[g].count, runs[g]);
     return t;
+}
+
+long m2l_split_elems(const std::vector<M2LSplit> &splits)
+{
+    long e = 0;
+    for (const M2LSplit &s : splits) e = std::max(e, s.part_off + s.parts * s.count);
+    return e;
 }
 
 void m2l_shape(int P, int mode, int &MP, int &KP)
@@ -471,8 +559,10 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     }
     const ST *Wg = static_cast<const ST *>(a.Wc) + task.c_off;
     [[maybe_unused]] const ST *Wg_lo = static_cast<const ST *>(a.Wc_lo) + task.c_off;
-    const int *cls_f = a.cls_f + task.cls * a.n_cls;
-    const int *cls_o = a.cls_o + task.cls * a.n_cls * D;
+    // a split level's task takes offsets [i0, i1) of its class
+    const int i0 = task.i1 ? task.i0 : 0, i1 = task.i1 ? task.i1 : a.n_cls;
+    const int *cls_f = a.cls_f + task.cls * a.n_cls + i0;
+    const int *cls_o = a.cls_o + (task.cls * a.n_cls + i0) * D;
 
     uint4 g[EPT / VEC];
     [[maybe_unused]] uint4 g_lo[MODE == 2 ? EPT / VEC : 1];
@@ -536,7 +626,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
                 for (int q = 0; q < 4; ++q) accd[r][cb][q] = 0.0;
     }
 
-    const int n = a.n_cls * NCH;
+    const int n = (i1 - i0) * NCH;
     auto store_tile = [&](ST *tile) {  // registers -> LDS tile
         ST *dst = tile + col * LDB + part * EPT;
         if constexpr (MODE == 2) {
@@ -633,7 +723,7 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         __syncthreads();
     }
     if (!active) return;
-    TW *Wl = static_cast<TW *>(a.Wl) + task.w_off;
+    TW *Wl = task.i1 ? static_cast<TW *>(a.Wpart) + task.o_off : static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
         const int cc = 16 * cb + (lane & 15);
@@ -654,22 +744,6 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
         }
     }
 }
-
-// The contraction chunk of a padded length: all of it below 128 (and at 128
-// for FP32 and BF16x3); FP64 at 128 in chunks of 64 (the smaller operand
-// tiles and fragments raise occupancy: 6.0 -> 5.0 ms at 5 nodes, N = 4M);
-// beyond 128, 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a
-// multiple of 128), so that a step's MFMA work still covers the next step's
-// loads.
-constexpr int m2l_chunk(int KP, int mode)
-{
-    return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 64 : 128) : 32;
-}
-
-// Row blocks of 16 per wave: 2 for BF16x3 at a padded length of 128 (one
-// workgroup covers all MP <= 128 rows; measured 1.15 -> 1.10 ms at 5 nodes,
-// N = 4M), else 1 (FP32 at 2 measured 2.7 -> 3.8 ms, FP64 unchanged).
-constexpr int m2l_row_blocks(int KP, int mode) { return KP == 128 && mode == 2 ? 2 : 1; }
 
 // Tile widths with a pipelined kernel for (MODE, KP, TB); the others use
 // m2l_mfma4 / m2l_bf16x3.
@@ -795,6 +869,47 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
     return FMMA_OK;
 }
 
+struct M2LSumArgs {
+    int n;
+    long start[kMaxLevels + 1];  // element ranges of the split levels, concatenated
+    long w_off[kMaxLevels], count[kMaxLevels], part_off[kMaxLevels];
+    int parts[kMaxLevels];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void m2l_sum_parts_kernel(T *Wl, const T *Wpart, M2LSumArgs s)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= s.start[s.n]) return;
+    int k = 0;
+    while (e >= s.start[k + 1]) ++k;
+    const long r = e - s.start[k];
+    const T *p = Wpart + s.part_off[k] + r;
+    T v = p[0];
+    for (int q = 1; q < s.parts[k]; ++q) v += p[(long)q * s.count[k]];
+    Wl[s.w_off[k] + r] = v;
+}
+
+template <typename T>
+int m2l_sum_parts(T *Wl, const T *Wpart, const std::vector<M2LSplit> &splits, hipStream_t st)
+{
+    if (splits.empty()) return FMMA_OK;
+    if ((int)splits.size() > kMaxLevels) return FMMA_ERR_INVALID;
+    M2LSumArgs s{};
+    s.n = (int)splits.size();
+    for (int k = 0; k < s.n; ++k) {
+        s.start[k + 1] = s.start[k] + splits[k].count;
+        s.w_off[k] = splits[k].w_off;
+        s.count[k] = splits[k].count;
+        s.part_off[k] = splits[k].part_off;
+        s.parts[k] = splits[k].parts;
+    }
+    const unsigned nb = (unsigned)((s.start[s.n] + 255) / 256);
+    hipLaunchKernelGGL((m2l_sum_parts_kernel<T>), dim3(nb), dim3(256), 0, st, Wl, Wpart, s);
+    FMMA_LAUNCHED("m2l_sum_parts");
+    return FMMA_OK;
+}
+
 // Multipoles of levels 2..L as MFMA operands, one row of KP (zero padded)
 // per box: TYPE (mode 0), FP32 (mode 1), or split BF16 hi = bf16(v),
 // lo = bf16(v - hi) of v = (float)W (mode 2).
@@ -829,6 +944,8 @@ int m2l_convert(const T *W, long nbox, int P, int KP, int mode, void *Wc, void *
     return FMMA_OK;
 }
 
+template int m2l_sum_parts<double>(double *, const double *, const std::vector<M2LSplit> &, hipStream_t);
+template int m2l_sum_parts<float>(float *, const float *, const std::vector<M2LSplit> &, hipStream_t);
 template int m2l_convert<double>(const double *, long, int, int, int, void *, void *, hipStream_t);
 template int m2l_convert<float>(const float *, long, int, int, int, void *, void *, hipStream_t);
 template int m2l_fragments<double>(const double *, int, int, int, void *, void *, hipStream_t);

@@ -33,7 +33,24 @@ struct M2LTask {
     unsigned p0;
     int np;
     long w_off, c_off, k_off;
+    // A split level (few workgroups, each a long chain of offset steps): the
+    // task takes the class's offsets [i0, i1) and writes its partial sums at
+    // element o_off of M2LKArgs::Wpart.  i1 == 0: every offset, into Wl.
+    int i0 = 0, i1 = 0;
+    long o_off = 0;
 };
+
+// A split level's partial sums: part p of the level's count = boxes x P
+// elements sits at part_off + p * count of Wpart; m2l_sum_parts adds the
+// parts in order into Wl at w_off.
+struct M2LSplit {
+    int level, parts;
+    long w_off, count, part_off;
+};
+
+// Workgroups a level's launch should reach before its offsets are split:
+// eight per CU.
+constexpr long kM2LSplitTarget = 2048;
 
 // A run of tasks with one tile width (one launch).
 struct M2LGroup {
@@ -45,9 +62,15 @@ struct M2LGroup {
 // the offsets are filled from per-level tables (element offsets) given by
 // the caller.  The tile width of a level is tb_of_level[l] when given and
 // positive, else the widest of 256/128/64 that keeps >= 512 workgroups.
+// With splits, a pipelined level whose launch would have at most half of
+// split_target workgroups is cut into 2..8 parts of at least 16 offsets each
+// (as many as keep the workgroups <= split_target); *splits lists them and
+// the elements of Wpart they take (return value of m2l_split_elems).
 std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std::vector<M2LGroup> &groups,
                                     const long *w_off, const long *c_off, const long *k_off,
-                                    const int *tb_of_level = nullptr, int only_level = -1);
+                                    const int *tb_of_level = nullptr, int only_level = -1,
+                                    std::vector<M2LSplit> *splits = nullptr, long split_target = 0);
+long m2l_split_elems(const std::vector<M2LSplit> &splits);
 bool m2l_pipe_supported(int D, int KP, int mode, int tb);
 
 // Kernel arguments of the MFMA contraction.
@@ -63,6 +86,7 @@ struct M2LKArgs {
     const void *Kf_lo;          // bf16 lo part
     const void *Wc;             // multipoles as MFMA operands, rows of KP: TYPE / float / bf16 hi
     const void *Wc_lo;          // bf16 lo part (mode 2)
+    void *Wpart;                // partial sums of split levels (TYPE)
 };
 
 // Shape of the operator fragments for a precision: rows MP, contraction KP.
@@ -80,6 +104,10 @@ int m2l_convert(const T *W, long nbox, int P, int KP, int mode, void *Wc, void *
 // One launch per task group: workgroup per task, blockDim 256, grid.y = row chunks.
 template <typename T, int D>
 int m2l_apply(const M2LKArgs &a, const std::vector<M2LGroup> &groups, int mode, hipStream_t st);
+
+// Wl of each split level = the sum of its parts, in part order.
+template <typename T>
+int m2l_sum_parts(T *Wl, const T *Wpart, const std::vector<M2LSplit> &splits, hipStream_t st);
 
 }  // namespace lib
 }  // namespace fmma

@@ -203,6 +203,8 @@ struct Plan {
     OffsetTables offs;
     int n_tasks = 0;
     std::vector<M2LGroup> groups;
+    std::vector<M2LSplit> splits;  // M2L levels whose offsets are split over several tasks
+    DBuf Wpart;                    // their partial sums
     long lvlc_off[kMaxLevels + 1] = {};
     long lvl_off[kMaxLevels + 1] = {};
     long kf_off[kMaxLevels] = {};
@@ -343,6 +345,10 @@ static int setup_basis(Plan &pl)
 }
 
 // ... and those that depend on the leaf level and the M2L mode.
+// Workgroups below which an M2L level's offsets are split over several
+// tasks (none in a no-alloc solve: its scratch holds no partial sums).
+static long split_target() { return t_arena ? 0 : kM2LSplitTarget; }
+
 static int setup(Plan &pl, int L)
 {
     const int n = pl.poly_ord + 1, D = pl.dim, P = pl.P(), mode = pl.mode();
@@ -376,7 +382,9 @@ static int setup(Plan &pl, int L)
             const size_t opsz = mode == 0 ? pl.tsize() : mode == 1 ? 4 : 2;
             if ((rc = pl.Wc.ensure(offc * opsz))) return rc;
             if (mode == 2 && (rc = pl.Wc_lo.ensure(offc * 2))) return rc;
-            std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off);
+            std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off,
+                                                        nullptr, -1, &pl.splits, split_target());
+            if ((rc = pl.Wpart.ensure((size_t)m2l_split_elems(pl.splits) * pl.tsize()))) return rc;
             record_tiles(pl, tasks, D);
             std::vector<int> op_levels, op_prob(L - 1, 0);
             for (int l = 2; l <= L; ++l) op_levels.push_back(l);
@@ -503,11 +511,15 @@ static int tune_single(Plan &pl, const M2LKArgs &ka, int L)
             int one[kMaxLevels] = {};
             one[l] = tb;
             std::vector<M2LGroup> g;
-            std::vector<M2LTask> t = make_m2l_tasks(D, L, P, mode, g, pl.lvl_off, pl.lvlc_off, pl.kf_off, one, l);
+            std::vector<M2LSplit> sp;
+            std::vector<M2LTask> t = make_m2l_tasks(D, L, P, mode, g, pl.lvl_off, pl.lvlc_off, pl.kf_off, one, l, &sp,
+                                                    split_target());
+            if ((rc = pl.Wpart.ensure((size_t)m2l_split_elems(sp) * pl.tsize()))) return rc;
             DBuf dt;
             if ((rc = upload(dt, t.data(), t.size() * sizeof(M2LTask), st))) return rc;
             M2LKArgs a2 = ka;
             a2.tasks = dt.as<M2LTask>();
+            a2.Wpart = pl.Wpart.p;
             if ((rc = m2l_apply<T, D>(a2, g, mode, st))) return rc;  // warm-up
             hipEvent_t e0, e1;
             FMMA_HIP(hipEventCreate(&e0));
@@ -529,7 +541,9 @@ static int tune_single(Plan &pl, const M2LKArgs &ka, int L)
         pl.tuned_tb_single[key] = best_tb;
         tb_of[l] = best_tb;
     }
-    std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off, tb_of);
+    std::vector<M2LTask> tasks = make_m2l_tasks(D, L, P, mode, pl.groups, pl.lvl_off, pl.lvlc_off, pl.kf_off, tb_of,
+                                                -1, &pl.splits, split_target());
+    if ((rc = pl.Wpart.ensure((size_t)m2l_split_elems(pl.splits) * pl.tsize()))) return rc;
     record_tiles(pl, tasks, D);
     pl.n_tasks = (int)tasks.size();
     if ((rc = upload(pl.d_tasks, tasks.data(), tasks.size() * sizeof(M2LTask), st))) return rc;
@@ -664,7 +678,9 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
             if ((pl.m2l_autotune || pl.forced_tb > 0) && pl.tuned_set_L != L && (rc = tune_single<T, D>(pl, ka, L)))
                 return rc;
             ka.tasks = pl.d_tasks.as<M2LTask>();
+            ka.Wpart = pl.Wpart.p;
             if ((rc = m2l_apply<T, D>(ka, pl.groups, mode, st))) return rc;
+            if ((rc = m2l_sum_parts<T>(Wl, pl.Wpart.as<T>(), pl.splits, st))) return rc;
         } else {
             for (int l = 2; l <= L; ++l) {
                 fmma_m2l_general_args ga{D, pl.dtype, n, P, pl.d_nodes.as<double>(), l,

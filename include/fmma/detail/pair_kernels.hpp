@@ -368,7 +368,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
 // box B - o_u (written by that box's wave, or zero by B's own wave when that
 // box is outside the grid or empty), slots[n_upper][i] point i's own sum.
 // p2p_combine (boxops.hip) adds the n_upper + 1 values of each point.
-constexpr int kP2PMutualTargets = 8;  // targets per lane per pass
+constexpr int kP2PMutualTargets = 10;  // targets per lane per pass: one pass up to 160 points
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v)
@@ -439,7 +439,7 @@ __device__ __forceinline__ void p2p_block_mutual_tpl(int tpl, const T *src, T *r
                                                      unsigned s, unsigned q, const T (&x)[NT][D], const T (&wt)[NT],
                                                      T (&acc)[NT], const K &kern)
 {
-    static_assert(NT == 8, "the switch covers 1..8 targets per lane");
+    static_assert(NT == 10, "the switch covers 1..10 targets per lane");
     switch (tpl) {
     case 1: p2p_block_mutual<S, 1>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
     case 2: p2p_block_mutual<S, 2>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
@@ -448,15 +448,17 @@ __device__ __forceinline__ void p2p_block_mutual_tpl(int tpl, const T *src, T *r
     case 5: p2p_block_mutual<S, 5>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
     case 6: p2p_block_mutual<S, 6>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
     case 7: p2p_block_mutual<S, 7>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
-    default: p2p_block_mutual<S, 8>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 8: p2p_block_mutual<S, 8>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    case 9: p2p_block_mutual<S, 9>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
+    default: p2p_block_mutual<S, 10>(src, rev, m, nch, s, q, x, wt, acc, kern); break;
     }
 }
 
-// Three waves per SIMD (at most 168 VGPRs) measured faster than the two the
-// compiler picks by itself for the 1/r kernel: 4.69 vs 4.76 ms at 128 points
-// per box, 1.33 vs 1.68 ms at 16.
+// Two waves per SIMD: with ten targets per lane the 1/r kernel then keeps
+// every value in registers (at three waves, 168 VGPRs, it spills), measured
+// 4.00-4.06 vs 4.06-4.14 ms at 128 points per box, N = 4M.
 template <typename T, int D, class K>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void p2p_mutual_kernel(fmma_p2p_args a,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void p2p_mutual_kernel(fmma_p2p_args a,
                                                                                                  K kern)
 {
     constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
