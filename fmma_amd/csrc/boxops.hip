@@ -461,43 +461,17 @@ __global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab, c
 // L2P over sorted targets: workgroup of 256 threads = 256 consecutive
 // targets.  Their boxes form one contiguous Morton range (targets are
 // sorted by box); when that range is short its locals are staged in LDS,
-// otherwise they are read from global memory.
+// otherwise they are read from global memory.  The two sources are separate
+// code paths: through one pointer that may be either, every load would be a
+// flat load waited for on its own.
 constexpr int kL2PElems = 2048;  // LDS coefficients staged per workgroup
 
-template <typename T, int D, int NN>
-__global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
+// The local expansion W (NN^D coefficients) at a point with Chebyshev
+// weights S, contracted one dimension at a time, innermost (last) first:
+// independent short sums instead of one chain over all coefficients.
+template <typename T, int D, int NN, typename PW>
+__device__ __forceinline__ T l2p_contract(PW W, const T (&S)[D][NN])
 {
-    constexpr int P = ipow_c(NN, D);
-    constexpr int NBX = kL2PElems / P > 0 ? kL2PElems / P : 1;
-    __shared__ T sc[NN * NN];
-    __shared__ T sW[NBX * P];
-    const long i0 = (long)blockIdx.x * 256;
-    const long i = i0 + threadIdx.x;
-    const long ilast = (i0 + 255 < a.n ? i0 + 255 : a.n - 1);
-    const unsigned kb0 = a.keys[i0], kb1 = a.keys[ilast];
-    const bool staged = kb1 - kb0 < (unsigned)NBX;
-    for (int e = threadIdx.x; e < NN * NN; e += 256) sc[e] = a.coef[e];
-    if (staged) {
-        const T *src = a.Wl + (size_t)kb0 * P;
-        const int cnt = (int)(kb1 - kb0 + 1) * P;
-        for (int e = threadIdx.x; e < cnt; e += 256) sW[e] = src[e];
-    }
-    __syncthreads();
-    if (i >= a.n) return;
-    const unsigned box = a.keys[i];
-    const T *W = staged ? sW + (size_t)(box - kb0) * P : a.Wl + (size_t)box * P;
-    const Geom<T> *gm;
-    const int NL = 1 << box_geom<T>(a, box, gm);
-    const T len = gm->len_leaf;
-    T S[D][NN];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        T rel;
-        leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
-        cheb_weights_ct<T, NN>(sc, rel, T(1), S[d]);
-    }
-    // the contraction one dimension at a time, innermost (last) first:
-    // independent short sums instead of one chain over all P coefficients
     T acc = T(0);
     if constexpr (D == 1) {
 #pragma unroll
@@ -524,7 +498,93 @@ __global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
             acc += S[0][a0] * u;
         }
     }
-    a.out[i] = acc;
+    return acc;
+}
+
+// The L2P value of sorted target i (i < a.n) of a workgroup whose locals
+// were staged (kb0 = the first box of the range) or not.
+template <typename T, int D, int NN>
+__device__ __forceinline__ T l2p_point(const L2PArgs<T> &a, long i, const T *sc, const T *sW, unsigned kb0,
+                                       bool staged)
+{
+    constexpr int P = ipow_c(NN, D);
+    const unsigned box = a.keys[i];
+    const Geom<T> *gm;
+    const int NL = 1 << box_geom<T>(a, box, gm);
+    const T len = gm->len_leaf;
+    T S[D][NN];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        T rel;
+        leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
+        cheb_weights_ct<T, NN>(sc, rel, T(1), S[d]);
+    }
+    // explicit address spaces: the two contractions cannot be merged into
+    // one over a generic pointer
+    typedef const __attribute__((address_space(3))) T *LdsPtr;
+    typedef const __attribute__((address_space(1))) T *GlobalPtr;
+    if (staged) return l2p_contract<T, D, NN>((LdsPtr)(sW + (size_t)(box - kb0) * P), S);
+    return l2p_contract<T, D, NN>((GlobalPtr)(a.Wl + (size_t)box * P), S);
+}
+
+// Stages the Chebyshev coefficients and, when the workgroup's boxes are few
+// enough, their locals; returns whether it staged them.
+template <typename T, int D, int NN, int NBX>
+__device__ __forceinline__ bool l2p_stage(const L2PArgs<T> &a, long i0, T *sc, T *sW, unsigned &kb0)
+{
+    constexpr int P = ipow_c(NN, D);
+    const long ilast = (i0 + 255 < a.n ? i0 + 255 : a.n - 1);
+    kb0 = a.keys[i0];
+    const unsigned kb1 = a.keys[ilast];
+    const bool staged = kb1 - kb0 < (unsigned)NBX;
+    for (int e = threadIdx.x; e < NN * NN; e += 256) sc[e] = a.coef[e];
+    if (staged) {
+        const T *src = a.Wl + (size_t)kb0 * P;
+        const int cnt = (int)(kb1 - kb0 + 1) * P;
+        for (int e = threadIdx.x; e < cnt; e += 256) sW[e] = src[e];
+    }
+    __syncthreads();
+    return staged;
+}
+
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(256) void l2p_ct(L2PArgs<T> a)
+{
+    constexpr int P = ipow_c(NN, D);
+    constexpr int NBX = kL2PElems / P > 0 ? kL2PElems / P : 1;
+    __shared__ T sc[NN * NN];
+    __shared__ T sW[NBX * P];
+    const long i0 = (long)blockIdx.x * 256;
+    const long i = i0 + threadIdx.x;
+    unsigned kb0;
+    const bool staged = l2p_stage<T, D, NN, NBX>(a, i0, sc, sW, kb0);
+    if (i >= a.n) return;
+    a.out[i] = l2p_point<T, D, NN>(a, i, sc, sW, kb0, staged);
+}
+
+// L2P fused with the mutual P2P form's combine pass: one thread per sorted
+// point computes its L2P value as l2p_ct does and adds the point's P2P
+// slots in p2p_combine_kernel's order, so ans is written once and the L2P
+// values make no round trip through memory.  Bit-identical to l2p_ct
+// followed by p2p_combine_kernel with base = the L2P values.
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(256) void l2p_combine_ct(L2PArgs<T> a, P2PCombineArgs<T> c)
+{
+    constexpr int P = ipow_c(NN, D);
+    constexpr int NBX = kL2PElems / P > 0 ? kL2PElems / P : 1;
+    constexpr int NU = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
+    __shared__ T sc[NN * NN];
+    __shared__ T sW[NBX * P];
+    const long i0 = (long)blockIdx.x * 256;
+    const long i = i0 + threadIdx.x;
+    unsigned kb0;
+    const bool staged = l2p_stage<T, D, NN, NBX>(a, i0, sc, sW, kb0);
+    if (i >= a.n) return;
+    T v = c.slots[(long)NU * c.n + i];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) v += c.slots[(long)u * c.n + i];
+    const T base = l2p_point<T, D, NN>(a, i, sc, sW, kb0, staged);
+    c.ans[c.perm[i]] = base + v;
 }
 
 #define FMMA_NN_SWITCH(NNV, CALL)                          \
@@ -564,6 +624,13 @@ static bool l2p_fast(const L2PArgs<T> &a, hipStream_t st)
 {
     const unsigned nb = (unsigned)((a.n + 255) / 256);
     FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((l2p_ct<T, D, NN>), dim3(nb), dim3(256), 0, st, a));
+}
+
+template <typename T, int D>
+static bool l2p_combine_fast(const L2PArgs<T> &a, const P2PCombineArgs<T> &c, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((a.n + 255) / 256);
+    FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((l2p_combine_ct<T, D, NN>), dim3(nb), dim3(256), 0, st, a, c));
 }
 
 // ----------------------------------------------------------- L2P gradient
@@ -758,8 +825,18 @@ int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st)
     return FMMA_OK;
 }
 
+template <typename T, int D>
+int l2p_combine(const L2PArgs<T> &a, const P2PCombineArgs<T> &c, hipStream_t st)
+{
+    if (a.n <= 0) return FMMA_OK;
+    if (a.n != c.n || !l2p_combine_fast<T, D>(a, c, st)) return FMMA_ERR_UNSUPPORTED;
+    FMMA_LAUNCHED("l2p_combine");
+    return FMMA_OK;
+}
+
 #define FMMA_BOX_INST(T, D)                                                                 \
     template int l2p_grad<T, D>(const L2PArgs<T> &, hipStream_t);                           \
+    template int l2p_combine<T, D>(const L2PArgs<T> &, const P2PCombineArgs<T> &, hipStream_t); \
     template int p2p_combine<T, D>(const P2PCombineArgs<T> &, hipStream_t);                 \
     template int p2m<T, D>(const P2MArgs<T> &, long, hipStream_t);                          \
     template int m2m<T, D>(const T *, T *, long, const T *, int, int, hipStream_t, const long *); \

@@ -73,6 +73,12 @@ int l2p(const L2PArgs<T> &a, long nbox, hipStream_t st);
 template <typename T, int D>
 int p2p_combine(const P2PCombineArgs<T> &a, hipStream_t st);
 
+// l2p and p2p_combine (with base = the L2P values) in one pass over the
+// sorted points, bit-identical to the two; FMMA_ERR_UNSUPPORTED (nothing
+// launched) for node counts without the compile-time kernels.
+template <typename T, int D>
+int l2p_combine(const L2PArgs<T> &a, const P2PCombineArgs<T> &c, hipStream_t st);
+
 // Gradient of the L2P interpolant with respect to the target coordinates:
 // a.out is [D][n], sorted order.
 template <typename T, int D>

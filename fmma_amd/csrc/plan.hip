@@ -760,10 +760,27 @@ static int fmm_eval(Plan &pl, T *ans)
     hipStream_t st = pl.stream;
     int rc;
     const T *base = nullptr;
+    L2PArgs<T> la{t_start, t_keys, t_pos, nt, pl.geom.as<Geom<T>>(), L, pl.d_coef.as<T>(), n, P, Wl + pl.lvl_off[L],
+                  nullptr};
+    const bool mutual = shared && pl.p2p_mutual && nbox > 1 && (pl.ops.symmetry == 1 || pl.ops.symmetry == -1);
+    if (L >= 2 && mutual && n >= 2 && n <= 8) {
+        // the mutual near field, then L2P inside its combine pass (stage "P2P sum")
+        pl.mark(6);
+        const int n_upper = ((D == 1 ? 3 : D == 2 ? 9 : 27) - 1) / 2;
+        if ((rc = pl.p2p_slots.ensure((size_t)(n_upper + 1) * nt * sizeof(T)))) return rc;
+        fmma_p2p_args qa{D, pl.dtype, (long)nbox, L, t_start, pl.s_start.as<unsigned>(), t_pos, pl.s_pos.as<T>(),
+                         pl.s_w.as<T>(), nt, ns, nullptr, t_perm, ans, 0, pl.p2p_slots.p, pl.ops.symmetry,
+                         nullptr, nullptr, nullptr, 0};
+        if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+        pl.mark(7);
+        P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), nt, nullptr, t_perm, ans, 0};
+        if ((rc = l2p_combine<T, D>(la, ca, st))) return rc;
+        pl.mark(8);
+        return FMMA_OK;
+    }
     if (L >= 2) {
         if ((rc = pl.l2p_out.ensure((size_t)nt * sizeof(T)))) return rc;
-        L2PArgs<T> la{t_start, t_keys, t_pos, nt, pl.geom.as<Geom<T>>(), L, pl.d_coef.as<T>(), n, P, Wl + pl.lvl_off[L],
-                      pl.l2p_out.as<T>()};
+        la.out = pl.l2p_out.as<T>();
         if ((rc = l2p<T, D>(la, nbox, st))) return rc;
         base = pl.l2p_out.as<T>();
     }
