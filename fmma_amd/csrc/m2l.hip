@@ -315,13 +315,14 @@ __device__ __forceinline__ void target_coords(const M2LTask &t, int tid, int (&t
 // LDS a pipelined M2L workgroup may take for two operand tiles.
 constexpr int kM2LDoubleLds = 160 * 1024 - 4096;
 
-// Row padding (elements) of an LDS operand tile [column][k]: one element
-// for FP64 and FP32 (an odd row length spreads the 16 columns of an MFMA
-// B-operand read over the banks; KC + 2 measured 5x the FP64 bank
-// conflicts of KC + 1, and KC + 4 did not change FP32's), 8 for BF16 (16-byte
-// rows of 16 columns on distinct banks).
+// Row padding (elements) of an LDS operand tile [column][k] so that the
+// lanes of an MFMA B-operand read (column = lane % 16, k = lane / 16 + a
+// step) hit distinct banks of the 32: one element for FP64 (8-byte reads of
+// 16 lanes at a time: word 2 * column, the 32 banks; KC + 2 measured 5x the
+// conflicts), two for FP32 (half a wave, columns 0..15 at k and k + 1:
+// bank 2 * column + k), 8 for BF16 (16-byte reads of 8 lanes: 4 banks each).
 template <int ESIZE>
-constexpr int kLdsPad = ESIZE == 2 ? 8 : 1;
+constexpr int kLdsPad = ESIZE == 2 ? 8 : ESIZE == 4 ? 2 : 1;
 
 // Operand rows staged per LDS pass of m2l_mfma4: 64 boxes x (256 + 1) FP64
 // values = 132 KB, inside the 160 KB of a CU for every P <= 512.
