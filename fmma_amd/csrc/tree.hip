@@ -126,6 +126,7 @@ __global__ __launch_bounds__(64) void bbox_final(const T *part, int np, Geom<T> 
 template <typename T, int D>
 int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_side, hipStream_t st)
 {
+    if (t == s && nt == ns) ns = 0;  // one point set: read it once
     const long n = nt > ns ? nt : ns;
     int nb = (int)((n + 255) / 256);
     nb = nb < kBBoxParts ? nb : kBBoxParts;
