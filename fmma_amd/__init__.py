@@ -129,6 +129,11 @@ class Plan:
         symmetric kernel each pair of neighbour boxes is evaluated once."""
         _lib.check(self._L.fmma_plan_set_p2p_mutual(self._h, 1 if on else 0))
 
+    def set_p2p_balance(self, on):
+        """Balance the direct P2P over work items when some target boxes have
+        a near field far above the average (default): clustered points."""
+        _lib.check(self._L.fmma_plan_set_p2p_balance(self._h, 1 if on else 0))
+
     # ---- solve (src/fmma.cpp:143-207)
     def solve(self, targets, weights, sources, ans):
         """ans <- sum_j w_j f(x_i, y_j).  weights may be None (all ones).

@@ -36,7 +36,7 @@ EXPORTS = [
     "fmma_plan_set_depth", "fmma_plan_get_depth", "fmma_plan_set_trans_sym_flag", "fmma_plan_set_nrn_N",
     "fmma_plan_get_nrn_N",
     "fmma_plan_set_kernel", "fmma_plan_set_kernel_ops", "fmma_plan_set_m2l_precision", "fmma_plan_set_stream",
-    "fmma_plan_set_timing", "fmma_plan_set_p2p_mutual", "fmma_plan_gradient", "fmma_plan_gradient_host",
+    "fmma_plan_set_timing", "fmma_plan_set_p2p_mutual", "fmma_plan_set_p2p_balance", "fmma_plan_gradient", "fmma_plan_gradient_host",
     "fmma_plan_solve_transpose", "fmma_plan_solve_transpose_host", "fmma_plan_solve_batch",
     "fmma_plan_set_batch_graph", "fmma_plan_set_m2l_autotune", "fmma_plan_set_m2l_tile", "fmma_plan_m2l_tiles", "fmma_plan_solve", "fmma_plan_solve_noalloc", "fmma_plan_solve_host", "fmma_plan_time_log",
     "fmma_plan_clear_time_log", "fmma_plan_level_coeffs", "fmma_plan_point_boxes", "fmma_morton_to_rowmajor",
@@ -93,6 +93,7 @@ def lib():
     L.fmma_plan_set_stream.argtypes = [vp, vp]
     L.fmma_plan_set_timing.argtypes = [vp, i]
     L.fmma_plan_set_p2p_mutual.argtypes = [vp, i]
+    L.fmma_plan_set_p2p_balance.argtypes = [vp, i]
     L.fmma_plan_gradient.argtypes = [vp, l, vp, vp, l, vp, vp, i]
     L.fmma_plan_gradient_host.argtypes = [vp, l, vp, vp, l, vp, vp]
     L.fmma_plan_solve_transpose.argtypes = [vp, l, vp, vp, l, vp, vp]

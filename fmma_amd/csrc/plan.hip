@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fmma/detail/pair_kernels.hpp"
 #include "batch.hpp"
 #include "boxops.hpp"
 #include "builtins.hpp"
@@ -203,6 +204,8 @@ struct Plan {
     OffsetTables offs;
     int n_tasks = 0;
     std::vector<M2LGroup> groups;
+    bool p2p_balance = true;       // direct P2P: split boxes with a near field far above the average
+    DBuf split_parts, split_cnt, split_items, split_part;
     std::vector<M2LSplit> splits;  // M2L levels whose offsets are split over several tasks
     DBuf Wpart;                    // their partial sums
     long lvlc_off[kMaxLevels + 1] = {};
@@ -717,6 +720,53 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
     return FMMA_OK;
 }
 
+// The direct P2P (dim_out 0) or its gradient (dim_out = D) balanced over
+// work items when some target boxes have a near field far above the
+// average (clustered points): the planner marks those boxes on the device,
+// one read-back gives the item count, then the unsplit boxes, the items and
+// the chunk sums run.  Without split boxes (or in a no-alloc solve) this is
+// the plain launch.  t_keys: the box of each sorted target.
+template <typename T, int D, class A, class Launch>
+static int balanced_p2p(Plan &pl, A qa, const unsigned *t_keys, unsigned pass, int dim_out, const T *base, T *out,
+                        int accumulate, Launch launch)
+{
+    const long nbox = qa.n_target_boxes;
+    if (t_arena || !pl.p2p_balance || !t_keys || nbox <= 1) return launch(qa);
+    hipStream_t st = pl.stream;
+    int rc;
+    if ((rc = pl.split_parts.ensure((size_t)nbox * 4)) || (rc = pl.split_cnt.ensure(16))) return rc;
+    unsigned *parts = pl.split_parts.as<unsigned>(), *cnt = pl.split_cnt.as<unsigned>();
+    FMMA_HIP(hipMemsetAsync(cnt, 0, 16, st));
+    const unsigned nb = (unsigned)((nbox + 255) / 256);
+    hipLaunchKernelGGL((detail::p2p_split_plan_kernel<D, A>), dim3(nb), dim3(256), 0, st, qa, pass, parts, cnt,
+                       (unsigned *)nullptr, 0);
+    FMMA_LAUNCHED("p2p_split_plan");
+    unsigned h[4];
+    FMMA_HIP(hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, st));
+    FMMA_HIP(hipStreamSynchronize(st));
+    if (h[0] == 0) return launch(qa);
+    const size_t per = (size_t)qa.n_t * (dim_out > 0 ? dim_out : 1) * sizeof(T);
+    if ((rc = pl.split_items.ensure((size_t)h[0] * 3 * 4)) || (h[1] > 1 && (rc = pl.split_part.ensure(per * h[1]))))
+        return rc;
+    hipLaunchKernelGGL((detail::p2p_split_plan_kernel<D, A>), dim3(nb), dim3(256), 0, st, qa, pass, parts, cnt,
+                       pl.split_items.as<unsigned>(), 1);
+    FMMA_LAUNCHED("p2p_split_plan");
+    qa.split_parts = parts;
+    if ((rc = launch(qa))) return rc;  // the boxes that were not split
+    A qi = qa;
+    qi.split_items = pl.split_items.as<unsigned>();
+    qi.n_split_items = (long)h[0];
+    qi.split_pass = pass;
+    qi.split_part = pl.split_part.p;
+    if ((rc = launch(qi))) return rc;
+    if (h[1] > 1) {
+        hipLaunchKernelGGL((detail::p2p_split_combine<T>), dim3((unsigned)((qa.n_t + 255) / 256)), dim3(256), 0, st,
+                           t_keys, parts, pl.split_part.as<T>(), qa.n_t, base, qa.t_perm, out, accumulate, dim_out);
+        FMMA_LAUNCHED("p2p_split_combine");
+    }
+    return FMMA_OK;
+}
+
 // P2P over the leaf neighbours (the tree's Morton boxes of leaf level L, or
 // with grid_n > 0 the row-major grid_n^D boxes): ans[perm[i]] = base[i] +
 // near sum (base NULL: the near sum, added onto ans when accumulate).  The
@@ -736,7 +786,14 @@ static int p2p_stage(Plan &pl, int L, int grid_n, unsigned nbox, const unsigned 
                      pl.s_w.as<T>(), nt, ns, base, t_perm, ans, accumulate,
                      mutual ? pl.p2p_slots.p : nullptr, mutual ? pl.ops.symmetry : 0,
                      nullptr, nullptr, nullptr, grid_n};
-    if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+    if (mutual) {
+        if ((rc = pl.ops.p2p(&qa, pl.ops.ctx, st))) return rc;
+    } else {
+        const unsigned *t_keys = shared ? pl.s_keys.as<unsigned>() : pl.t_keys.as<unsigned>();
+        auto launch = [&](const fmma_p2p_args &x) { return pl.ops.p2p(&x, pl.ops.ctx, st); };
+        if ((rc = balanced_p2p<T, D>(pl, qa, t_keys, 64u * detail::kP2PTargets, 0, base, ans, accumulate, launch)))
+            return rc;
+    }
     if (mark_mid >= 0) pl.mark(mark_mid);
     if (mutual) {
         P2PCombineArgs<T> ca{pl.p2p_slots.as<T>(), nt, base, t_perm, ans, accumulate};
@@ -820,7 +877,8 @@ static int fmm_eval_grad(Plan &pl, T *grad)
     }
     fmma_p2p_grad_args ga{D, pl.dtype, (long)nbox, L, v.t_start, pl.s_start.as<unsigned>(), v.t_pos,
                           pl.s_pos.as<T>(), pl.s_w.as<T>(), nt, ns, base, v.t_perm, grad};
-    return pl.ops.p2p_grad(&ga, pl.ops.ctx, st);
+    auto launch = [&](const fmma_p2p_grad_args &x) { return pl.ops.p2p_grad(&x, pl.ops.ctx, st); };
+    return balanced_p2p<T, D>(pl, ga, v.t_keys, 64u * detail::kP2PGradTargets, D, base, grad, 0, launch);
 }
 
 template <typename T, int D>
@@ -1840,6 +1898,13 @@ int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable)
 {
     if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
     plan->pl.p2p_mutual = enable != 0;
+    return FMMA_OK;
+}
+
+int fmma_plan_set_p2p_balance(fmma_plan plan, int enable)
+{
+    if (!plan) return fmma::lib::fail(FMMA_ERR_INVALID, "null plan");
+    plan->pl.p2p_balance = enable != 0;
     return FMMA_OK;
 }
 

@@ -220,6 +220,46 @@ __device__ __forceinline__ void p2p_block(const T *src, unsigned cnt, unsigned n
     p2p_range<TPL, true>(src, m, cnt, q, nslice, x, acc, kern);
 }
 
+// The work of one workgroup of the direct P2P kernels: a whole target box,
+// or one item (box, pass, chunk) of a box the library split (balanced
+// form); false when there is nothing to do.
+template <class A>
+__device__ __forceinline__ bool p2p_work_item(const A &a, unsigned &box, unsigned &t0, unsigned &t1, unsigned &chunk,
+                                              unsigned &nchunk)
+{
+    chunk = 0;
+    nchunk = 1;
+    if (a.split_items) {
+        const unsigned *it = a.split_items + 3 * (size_t)blockIdx.x;
+        box = it[0];
+        chunk = it[2];
+        nchunk = a.split_parts[box];
+        const unsigned tb0 = a.t_start[box], tb1 = a.t_start[box + 1];
+        t0 = tb0 + it[1] * a.split_pass;
+        t1 = t0 + a.split_pass < tb1 ? t0 + a.split_pass : tb1;
+        return t0 < t1;
+    }
+    box = blockIdx.x;
+    if (a.split_parts && a.split_parts[box]) return false;  // the item launch covers it
+    t0 = a.t_start[box];
+    t1 = a.t_start[box + 1];
+    return t0 < t1;
+}
+
+// Records [r_lo, r_hi) of chunk c of nchunk equal chunks of total records.
+__device__ __forceinline__ void chunk_range(unsigned total, unsigned c, unsigned nchunk, unsigned &r_lo,
+                                            unsigned &r_hi)
+{
+    if (nchunk <= 1) {
+        r_lo = 0;
+        r_hi = total;
+        return;
+    }
+    const unsigned cb = (total + nchunk - 1) / nchunk;
+    r_lo = c * cb < total ? c * cb : total;
+    r_hi = r_lo + cb < total ? r_lo + cb : total;
+}
+
 template <typename T, int D, class K>
 __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
 {
@@ -228,9 +268,8 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
     __shared__ __align__(16) T src[kP2PBlock * RW];
     __shared__ unsigned nb_first[NNB], nb_pre[NNB + 1];
 
-    const unsigned box = blockIdx.x;
-    const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
-    if (t0 == t1) return;
+    unsigned box, t0, t1, chunk, nchunk;
+    if (!p2p_work_item(a, box, t0, t1, chunk, nchunk)) return;
     const unsigned lane = threadIdx.x;
     const T *tp = static_cast<const T *>(a.t_pos);
     const T *sp = static_cast<const T *>(a.s_pos);
@@ -271,6 +310,8 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
     if (lane == 0) nb_pre[0] = 0;
     __syncthreads();
     const unsigned total = nb_pre[NNB], nself = nb_pre[1];
+    unsigned r_lo, r_hi;
+    chunk_range(total, chunk, nchunk, r_lo, r_hi);
 
     // slots: the fewest of 4, 8, .., 64 with <= kP2PTargets targets per lane
     const unsigned ntb = t1 - t0;
@@ -290,8 +331,8 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
 #pragma unroll
             for (int d = 0; d < D; ++d) x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
         }
-        for (unsigned base = 0; base < total; base += kP2PBlock) {
-            const unsigned cnt = total - base < (unsigned)kP2PBlock ? total - base : (unsigned)kP2PBlock;
+        for (unsigned base = r_lo; base < r_hi; base += kP2PBlock) {
+            const unsigned cnt = r_hi - base < (unsigned)kP2PBlock ? r_hi - base : (unsigned)kP2PBlock;
             __syncthreads();  // previous block fully consumed
             int b = 0;
             for (unsigned e = lane; e < cnt; e += 64) {
@@ -334,6 +375,10 @@ __global__ __launch_bounds__(64) void p2p_kernel(fmma_p2p_args a, K kern)
             for (int k = 0; k < kP2PTargets; ++k) {
                 const unsigned i = pass0 + s + nslot * k;
                 if (k >= tpl || i >= t1) continue;
+                if (nchunk > 1) {  // one chunk of a split box: its partial sum
+                    static_cast<T *>(a.split_part)[(long)chunk * nt + i] = acc[k];
+                    continue;
+                }
                 T *ans = static_cast<T *>(a.ans);
                 const unsigned oi = a.t_perm[i];
                 if (a.base)
@@ -620,9 +665,111 @@ int launch_p2p(const fmma_p2p_args *a, const K &k, hipStream_t s)
             return FMMA_ERR_INVALID;
         }
     } else {
-        hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+        const long nwg = a->split_items ? a->n_split_items : a->n_target_boxes;
+        if (nwg <= 0) return FMMA_OK;
+        hipLaunchKernelGGL((p2p_kernel<T, D, K>), dim3((unsigned)nwg), dim3(64), 0, s, *a, k);
     }
     return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
+}
+
+// ------------------------------------------------- balanced direct P2P
+//
+// With clustered points one target box can have hundreds of times the
+// average near field, and its single workgroup then sets the kernel's time.
+// The planner gives every box whose targets x neighbourhood records exceed
+// kP2PSplitWork pair evaluations a set of work items: passes of at most
+// `pass` targets, times chunks of the neighbourhood records sized so that
+// an item does about kP2PSplitWork evaluations (at most kP2PSplitMaxChunks).
+// Phase 0 marks the split boxes (parts = their chunk count) and counts the
+// items and the largest chunk count; phase 1, once the library has sized
+// the item list, writes the items.  Item order does not matter: every item
+// writes its own sums, and the chunks of a target are added in chunk order.
+constexpr unsigned long long kP2PSplitWork = 1ull << 21;
+constexpr unsigned kP2PSplitMaxChunks = 64;
+
+template <int D, class A>
+__global__ __launch_bounds__(256) void p2p_split_plan_kernel(A a, unsigned pass, unsigned *parts, unsigned *counters,
+                                                             unsigned *items, int phase)
+{
+    constexpr int NNB = D == 1 ? 3 : D == 2 ? 9 : 27;
+    const long box = (long)blockIdx.x * 256 + threadIdx.x;
+    if (box >= a.n_target_boxes) return;
+    const unsigned ntb = a.t_start[box + 1] - a.t_start[box];
+    unsigned total = 0;
+    if (ntb > 0) {
+        int bc[D];
+        unsigned gbase;
+        const int NL = box_coords_of<D>(a, (unsigned)box, bc, gbase);
+        for (int k = 0; k < NNB; ++k) {
+            int m[D], kk = k;
+            bool ok = true;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+                m[d] = bc[d] + (kk % 3) - 1;
+                kk /= 3;
+                ok = ok && m[d] >= 0 && m[d] < NL;
+            }
+            if (ok) {
+                const unsigned mm = box_index_of<D>(a, m, gbase);
+                total += a.s_start[mm + 1] - a.s_start[mm];
+            }
+        }
+    }
+    const unsigned long long work = (unsigned long long)ntb * total;
+    if (work <= kP2PSplitWork) {
+        if (phase == 0) parts[box] = 0;
+        return;
+    }
+    const unsigned npass = (ntb + pass - 1) / pass, tp = ntb < pass ? ntb : pass;
+    unsigned long long nch = ((unsigned long long)tp * total + kP2PSplitWork - 1) / kP2PSplitWork;
+    nch = nch < 1 ? 1 : nch > kP2PSplitMaxChunks ? kP2PSplitMaxChunks : nch;
+    const unsigned nitem = npass * (unsigned)nch;
+    if (phase == 0) {
+        parts[box] = (unsigned)nch;
+        atomicAdd(&counters[0], nitem);
+        atomicMax(&counters[1], (unsigned)nch);
+    } else {
+        const unsigned q = atomicAdd(&counters[2], nitem);
+        for (unsigned p = 0; p < npass; ++p)
+            for (unsigned c = 0; c < (unsigned)nch; ++c) {
+                unsigned *it = items + 3 * (size_t)(q + p * (unsigned)nch + c);
+                it[0] = (unsigned)box;
+                it[1] = p;
+                it[2] = c;
+            }
+    }
+}
+
+// The chunk sums of split boxes' targets, in chunk order (thread per sorted
+// target; keys = its box): a solve writes base + sum (or adds the sum to ans
+// or overwrites, as the P2P pass would), a gradient (dim > 0) writes the D
+// components base[d] + sum.
+template <typename T>
+__global__ __launch_bounds__(256) void p2p_split_combine(const unsigned *keys, const unsigned *parts, const T *part,
+                                                         long n, const T *base, const unsigned *perm, T *out,
+                                                         int accumulate, int dim)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned nch = parts[keys[i]];
+    if (nch <= 1) return;
+    const long oi = perm[i];
+    if (dim == 0) {
+        T v = part[i];
+        for (unsigned c = 1; c < nch; ++c) v += part[(long)c * n + i];
+        if (base)
+            out[oi] = base[i] + v;
+        else if (accumulate)
+            out[oi] += v;
+        else
+            out[oi] = v;
+        return;
+    }
+    for (int d = 0; d < dim; ++d) {
+        T v = part[(long)d * n + i];
+        for (unsigned c = 1; c < nch; ++c) v += part[((long)c * dim + d) * n + i];
+        out[oi * dim + d] = base ? base[(long)d * n + i] + v : v;
+    }
 }
 
 // ------------------------------------------------------ P2P gradient
@@ -672,9 +819,8 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
     __shared__ __align__(16) T src[kP2PBlock * RW];
     __shared__ unsigned nb_first[NNB], nb_pre[NNB + 1];
 
-    const unsigned box = blockIdx.x;
-    const unsigned t0 = a.t_start[box], t1 = a.t_start[box + 1];
-    if (t0 == t1) return;
+    unsigned box, t0, t1, chunk, nchunk;
+    if (!p2p_work_item(a, box, t0, t1, chunk, nchunk)) return;
     const unsigned lane = threadIdx.x;
     const T *tp = static_cast<const T *>(a.t_pos);
     const T *sp = static_cast<const T *>(a.s_pos);
@@ -716,6 +862,8 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
     if (lane == 0) nb_pre[0] = 0;
     __syncthreads();
     const unsigned total = nb_pre[NNB], nself = nb_pre[1];
+    unsigned r_lo, r_hi;
+    chunk_range(total, chunk, nchunk, r_lo, r_hi);
 
     const unsigned ntb = t1 - t0;
     unsigned nslot = 4;
@@ -736,8 +884,8 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
                 x[k][d] = k < tpl ? tp[d * nt + ic] : T(0);
             }
         }
-        for (unsigned b0 = 0; b0 < total; b0 += kP2PBlock) {
-            const unsigned cnt = total - b0 < (unsigned)kP2PBlock ? total - b0 : (unsigned)kP2PBlock;
+        for (unsigned b0 = r_lo; b0 < r_hi; b0 += kP2PBlock) {
+            const unsigned cnt = r_hi - b0 < (unsigned)kP2PBlock ? r_hi - b0 : (unsigned)kP2PBlock;
             __syncthreads();
             int b = 0;
             for (unsigned e = lane; e < cnt; e += 64) {
@@ -778,6 +926,12 @@ __global__ __launch_bounds__(64) void p2p_grad_kernel(fmma_p2p_grad_args a, K ke
             for (int k = 0; k < MT; ++k) {
                 const unsigned i = pass0 + s + nslot * k;
                 if (k >= tpl || i >= t1) continue;
+                if (nchunk > 1) {  // one chunk of a split box: its partial sums
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+                        static_cast<T *>(a.split_part)[((long)chunk * D + d) * nt + i] = acc[k][d];
+                    continue;
+                }
                 const long oi = a.t_perm[i];
 #pragma unroll
                 for (int d = 0; d < D; ++d) grad[oi * D + d] = base ? base[d * nt + i] + acc[k][d] : acc[k][d];
@@ -790,8 +944,9 @@ template <typename T, int D, class K>
 int launch_p2p_grad(const fmma_p2p_grad_args *a, const K &k, hipStream_t s)
 {
     if constexpr (has_grad<K, T, D>::value) {
-        if (a->n_target_boxes <= 0) return FMMA_OK;
-        hipLaunchKernelGGL((p2p_grad_kernel<T, D, K>), dim3((unsigned)a->n_target_boxes), dim3(64), 0, s, *a, k);
+        const long nwg = a->split_items ? a->n_split_items : a->n_target_boxes;
+        if (nwg <= 0) return FMMA_OK;
+        hipLaunchKernelGGL((p2p_grad_kernel<T, D, K>), dim3((unsigned)nwg), dim3(64), 0, s, *a, k);
         return hipGetLastError() == hipSuccess ? FMMA_OK : FMMA_ERR_HIP;
     } else {
         return FMMA_ERR_UNSUPPORTED;

@@ -100,6 +100,23 @@ typedef struct {
     /* > 0: the leaf boxes are a row-major grid of grid_n per side (the nrnmm
      * solver) instead of a Morton-ordered 2^leaf_level tree. */
     int grid_n;
+    /* Balanced direct form (slots == NULL), for boxes whose near field is
+     * far above the average (clustered points).  split_parts[box] > 0 marks a
+     * box the library split into work items; a launch without split_items
+     * skips those boxes.  A launch with split_items runs one workgroup per
+     * item split_items[3q .. 3q+2] = (box, pass, chunk): the box's targets
+     * t_start[box] + pass * split_pass .. (at most split_pass of them)
+     * against chunk `chunk` of the concatenated neighbourhood records (the
+     * box's own sources first, then the other neighbour boxes in row-major
+     * order) cut into split_parts[box] chunks of equal size.  With more than
+     * one chunk the item writes its sums to split_part[chunk * n_t + i]
+     * (sorted order) and leaves ans alone; the library adds the chunks of a
+     * target in chunk order. */
+    const unsigned *split_parts;  /* [n_boxes], or NULL */
+    const unsigned *split_items;  /* [n_split_items][3], or NULL */
+    long n_split_items;
+    unsigned split_pass;
+    void *split_part;             /* [max parts][n_t] TYPE */
 } fmma_p2p_args;
 
 typedef struct {
@@ -134,6 +151,13 @@ typedef struct {
     const unsigned *problem_box_base;
     const int *problem_leaf_level;
     int grid_n;                   /* as in fmma_p2p_args */
+    /* The balanced form as in fmma_p2p_args; chunk sums at
+     * split_part[(chunk * dim + d) * n_t + i]. */
+    const unsigned *split_parts;
+    const unsigned *split_items;
+    long n_split_items;
+    unsigned split_pass;
+    void *split_part;             /* [max parts][dim][n_t] TYPE */
 } fmma_p2p_grad_args;
 
 /* Far field of the tree and nrnmm solvers (M2P, src/fmm_core.cpp:625-661 and
@@ -298,6 +322,14 @@ int fmma_plan_set_timing(fmma_plan plan, int enable);
  * needs (3^dim+1)/2 * n TYPE values of scratch.  The reference evaluates every
  * target box against its whole neighbourhood (src/fmm.cpp:122-141). */
 int fmma_plan_set_p2p_mutual(fmma_plan plan, int enable);
+
+/* New: balance the direct P2P and its gradient over work items when some
+ * target boxes have far more near-field pairs than the average (default 1;
+ * clustered points): such a box's targets are cut into passes and its
+ * neighbourhood sources into chunks whose sums are added in a fixed order.
+ * Costs one small device-to-host read per direct P2P.  The reference runs
+ * one loop per target box (src/fmm.cpp:122-141). */
+int fmma_plan_set_p2p_balance(fmma_plan plan, int enable);
 
 /* ------------------------------------------------------------------ solve */
 
