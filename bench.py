@@ -109,16 +109,22 @@ def m2l_pairs(depth, dim=3):
     return sum((6 * 2 ** l - 8) ** dim - (3 * 2 ** l - 2) ** dim for l in range(2, depth))
 
 
-def measured_traffic(kernel, n, depth):
+def measured_traffic(kernel, n, depth, poly_ord, form="fp32"):
     """HBM-side bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled
     by the plain-copy calibration) from the committed rocprofv3 --pmc summary
-    of this workload, or None when the summary does not cover this run."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_3d_n4m_depth6.md")
-    if n != 1 << 22 or depth != 6 or not os.path.exists(path):
+    of this workload (the table of the given M2L form), or None when the
+    summary does not cover this run."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_3d_n4m_depth6_nodes5.md")
+    if n != 1 << 22 or depth != 6 or poly_ord != 4 or not os.path.exists(path):
         return None
+    head = {"fp32": "FP32 M2L", "native": "FP64 (native)", "bf16x3": "BF16x3 M2L"}[form]
+    inside = False
     for line in open(path):
+        if line.startswith("###"):
+            inside = head in line
+            continue
         cells = [c.strip().strip("`") for c in line.split("|")]
-        if len(cells) > 5 and kernel in cells[1]:
+        if inside and len(cells) > 5 and kernel in cells[1]:
             return int(cells[4]) + int(cells[5])
     return None
 
@@ -628,14 +634,14 @@ def config4_bench(fmma_amd, torch, steps):
     return out
 
 
-def roofline_of(stages, dim_pairs, flops_m2l, form, n, depth, p2p_kernel="p2p_mutual_kernel"):
+def roofline_of(stages, dim_pairs, flops_m2l, form, n, depth, poly_ord, p2p_kernel="p2p_mutual_kernel"):
     """Roofline of the dominant stage of one solve (HIP-event stage times)."""
     pairs, evaluated = dim_pairs
     dom = max(stages, key=stages.get)
     if dom == "P2P":
         ach = pairs * P2P_FLOPS_PER_PAIR / (stages["P2P"] * 1e-3)
         roof = {"kernel": p2p_kernel, "bound": "fp64-valu", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": measured_traffic(p2p_kernel, n, depth),
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": measured_traffic(p2p_kernel, n, depth, poly_ord, form),
                 "work": "%d pair interactions x %d flops" % (pairs, P2P_FLOPS_PER_PAIR),
                 "pairs_per_s": pairs / (stages["P2P"] * 1e-3), "evaluated_pairs": evaluated,
                 "evaluations_per_s": evaluated / (stages["P2P"] * 1e-3)}
@@ -643,7 +649,7 @@ def roofline_of(stages, dim_pairs, flops_m2l, form, n, depth, p2p_kernel="p2p_mu
         ach = flops_m2l / (stages["M2L"] * 1e-3)
         peak = PEAK_OF_FORM[form]
         roof = {"kernel": "m2l_pipe", "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": measured_traffic("m2l_pipe", n, depth),
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": measured_traffic("m2l_pipe", n, depth, poly_ord, form),
                 "work": "2*P^2 x M2L pairs = %.4g flop" % flops_m2l}
     else:
         nbytes = n * 4 * 8
@@ -727,7 +733,7 @@ def main():
     pairs, self_pairs = pair_count(x_np, depth)
     evaluated = self_pairs + (pairs - self_pairs) // 2  # the mutual form: each distinct box pair once
     P = (p_head + 1) ** 3
-    roof = roofline_of(stages, (pairs, evaluated), 2.0 * P * P * m2l_pairs(depth), head, n, depth)
+    roof = roofline_of(stages, (pairs, evaluated), 2.0 * P * P * m2l_pairs(depth), head, n, depth, p_head)
 
     config2 = None if args.no_config2 else config2_bench(fmma_amd, torch, args.steps, args.warmup, forms)
     sweep = None if args.no_sweep else published_sweep(fmma_amd, torch)
