@@ -100,8 +100,9 @@ constexpr int m2l_row_blocks(int KP, int mode) { return KP == 128 && mode == 2 ?
 // the contiguous range m2l_pipe gives each XCD (below) holds an equal share
 // of every run, each share contiguous in the original order (neighbouring
 // tiles on one XCD share partners in its L2).  Run remainders are dealt to
-// the XCDs cyclically, which gives XCD x exactly the nt8 + (x < rem) tasks
-// the kernel's mapping assigns it.
+// the XCDs cyclically, which gives XCD x the nt8 + (x < rem) tasks the
+// kernel's mapping assigns it when every task is one workgroup (more row
+// chunks per task shift the boundaries by at most one task).
 static void xcd_balance(std::vector<M2LTask> &t, int first, int count, const std::vector<int> &runs)
 {
     std::vector<M2LTask> per[8];
@@ -122,7 +123,8 @@ static void xcd_balance(std::vector<M2LTask> &t, int first, int count, const std
         for (const M2LTask &k : per[x]) t[o++] = k;
 }
 
-// Tasks level by level, deepest first.  A level's tile width is the widest
+// Tasks level by level, deepest first (inside a level in the order chosen
+// below).  A level's tile width is the widest
 // of 256/128/64 boxes that still gives >= 512 workgroups (two per CU), when a
 // pipelined kernel exists for it; tasks of equal width form one launch.  A
 // split level's parts of one tile are consecutive tasks (the same XCD).
@@ -143,6 +145,13 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
     n_cls -= n3;  // interaction offsets per class
     const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
     const long gy = (MP / 16 + 4 * rb - 1) / (4 * rb);  // workgroups per task (row chunks)
+    // Task order inside a level: tile-major (a tile's 2^D classes together)
+    // or class-major (all tiles of one class together, so that each XCD's
+    // contiguous share holds few classes and its tasks read the same
+    // operator rows at about the same time).  Class-major halves the BF16x3
+    // contraction's fabric reads at 5 nodes, N = 4M (6.5 -> 3.3 GB; 1.18 ->
+    // 1.01 ms); FP64 and FP32 lose with it (4.39 -> 4.54 ms, 2.32 -> 2.33 ms).
+    const bool class_major = mode == 2;
     long part_off = 0;
     std::vector<std::vector<int>> runs;  // per group: the first task of each level
     for (int l = leaf_level; l >= 2; --l) {
@@ -171,20 +180,22 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
             parts = p >= 2 ? (int)p : 1;
         }
         const long count = (long)P << (D * l);
-        for (unsigned p0 = 0; p0 < npar; p0 += tb)
-            for (int c = 0; c < (1 << D); ++c) {
-                const unsigned np = npar - p0 < (unsigned)tb ? npar - p0 : (unsigned)tb;
-                for (int q = 0; q < parts; ++q) {
-                    M2LTask k{l, c, p0, (int)np, w_off[l], c_off[l], k_off[l]};
-                    if (parts > 1) {
-                        k.i0 = q * n_cls / parts;
-                        k.i1 = (q + 1) * n_cls / parts;
-                        k.o_off = part_off + q * count;
-                    }
-                    t.push_back(k);
-                    ++groups.back().count;
+        const int nbg = (int)((npar + tb - 1) / tb);  // tiles per class
+        for (int it = 0; it < nbg << D; ++it) {
+            const int c = class_major ? it / nbg : it % (1 << D);
+            const unsigned p0 = (unsigned)(class_major ? it % nbg : it >> D) * tb;
+            const unsigned np = npar - p0 < (unsigned)tb ? npar - p0 : (unsigned)tb;
+            for (int q = 0; q < parts; ++q) {
+                M2LTask k{l, c, p0, (int)np, w_off[l], c_off[l], k_off[l]};
+                if (parts > 1) {
+                    k.i0 = q * n_cls / parts;
+                    k.i1 = (q + 1) * n_cls / parts;
+                    k.o_off = part_off + q * count;
                 }
+                t.push_back(k);
+                ++groups.back().count;
             }
+        }
         if (parts > 1) {
             splits->push_back(M2LSplit{l, parts, w_off[l], count, part_off});
             part_off += parts * count;
@@ -536,13 +547,16 @@ __global__ __launch_bounds__(256) void m2l_pipe(M2LKArgs a)
     __shared__ __align__(16) ST Bs[(DB ? 2 : 1) * BUF];
 
     // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
-    // contiguous run of tasks (neighbouring tiles, all classes) so that its
-    // L2 holds the multipoles and operators those tasks share.
-    const unsigned nt8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
-    const unsigned tix = xcd * nt8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    // contiguous run of (task, row chunk) pairs, the row chunks of a task
+    // next to each other, so that its L2 serves the partner tiles the chunks
+    // of one task share and the operator rows the tasks of one class share.
+    const unsigned gy = (unsigned)((a.MP / 16 + 4 * RB - 1) / (4 * RB));  // row chunks per task
+    const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
+    const unsigned v = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const unsigned tix = v / gy, chunk = v - tix * gy;
     const M2LTask task = a.tasks[tix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int rb0 = (blockIdx.y * 4 + wave) * RB;
+    const int rb0 = ((int)chunk * 4 + wave) * RB;
     bool act[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) act[r] = (rb0 + r) * 16 < a.MP;
@@ -758,8 +772,8 @@ static bool launch_pipe_tb(const M2LKArgs &a, dim3 grid, hipStream_t st)
                   (MODE == 2 && KP > 128 && TB > 128)) {
         return false;
     } else {
-        const dim3 g2(grid.x, (unsigned)((a.MP / 16 + 4 * RB - 1) / (4 * RB)));
-        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC, RB>), g2, dim3(256), 0, st, a);
+        const dim3 g1(grid.x * (unsigned)((a.MP / 16 + 4 * RB - 1) / (4 * RB)));
+        hipLaunchKernelGGL((m2l_pipe<T, D, MODE, KP, TB, KC, RB>), g1, dim3(256), 0, st, a);
         return true;
     }
 }
