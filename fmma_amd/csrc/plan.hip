@@ -186,7 +186,7 @@ struct Plan {
     DBuf d_nodes, d_coef, d_tab, d_far, d_cls_f, d_cls_o, d_tasks, d_op_levels, d_op_prob, d_pairs;
     int n_pairs = 0;
     DBuf h_t, h_s, h_w, h_ans;
-    DBuf rec;  // packed point records (coordinates, weight) for the gather after the sort
+    DBuf pos_alt;  // SoA coordinates and weights between the passes of the point sort
 
     // batches: the layout, graph replay, the M2L tile autotuner and its choices
     std::unique_ptr<BatchPlan> batch;
@@ -470,17 +470,14 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
         (rc = pos.ensure((size_t)n * D * sizeof(T))) || (rc = start.ensure(((size_t)nbox + 1) * 4)))
         return rc;
     if (with_w && (rc = wbuf->ensure((size_t)n * sizeof(T)))) return rc;
-    if ((rc = pl.rec.ensure((size_t)kRecStride<D> * n * sizeof(T)))) return rc;
+    if ((rc = pl.pos_alt.ensure((size_t)(D + 1) * n * sizeof(T)))) return rc;
     const Geom<T> *g = pl.geom.as<Geom<T>>();
-    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream, with_w ? w : nullptr,
-                              pl.rec.as<T>())))
-        return rc;
-    SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
-                   hist.as<unsigned>()};
-    if ((rc = radix_sort(sb, n, bits, pl.stream))) return rc;
-    if ((rc = box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream))) return rc;
-    return gather_records<T, D>(pl.rec.as<T>(), vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr,
-                                pl.stream);
+    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream))) return rc;
+    PointSortBuffers<T> sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(),
+                           vals_alt.as<unsigned>(), hist.as<unsigned>(), pos.as<T>(),
+                           with_w ? wbuf->as<T>() : nullptr, pl.pos_alt.as<T>(), pl.pos_alt.as<T>() + (size_t)D * n};
+    if ((rc = sort_points<T, D>(sb, n, bits, pts, w, with_w, pl.stream))) return rc;
+    return box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream);
 }
 
 // The M2L tile autotuner for one problem: per level, time every pipelined
@@ -1328,6 +1325,7 @@ static int batch_buffers(Plan &pl, BatchPlan &bp)
     if (!bp.shared && (rc = tree_bufs(bp.nt, pl.t_keys, pl.t_vals, pl.t_keys_alt, pl.t_vals_alt, pl.t_hist, pl.t_pos,
                                       nullptr, pl.t_start)))
         return rc;
+    if ((rc = pl.pos_alt.ensure((size_t)(D + 1) * std::max(bp.ns, bp.nt) * ts))) return rc;
     const size_t opsz = mode == 0 ? ts : mode == 1 ? 4 : 2;
     const size_t esz = mode == 0 ? 8 : mode == 1 ? 4 : 2;
     const size_t fe = (size_t)pl.offs.n_far * MP * KP;
@@ -1352,11 +1350,11 @@ static int batch_tree(Plan &pl, BatchPlan &bp, const T *pts, const T *w, long n,
     if ((rc = batch_keys<T, D>(pts, n, d_off, bp.nprob, bp.d_geoms.as<Geom<T>>(), bp.d_levels.as<int>(),
                                bp.d_bbase.as<unsigned>(), keys.as<unsigned>(), st)))
         return rc;
-    SortBuffers sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(), vals_alt.as<unsigned>(),
-                   hist.as<unsigned>()};
-    if ((rc = radix_sort(sb, n, bp.bits, st))) return rc;
-    if ((rc = box_ranges(keys.as<unsigned>(), n, (unsigned)bp.NB, start.as<unsigned>(), st))) return rc;
-    return gather<T, D>(pts, w, vals.as<unsigned>(), n, pos.as<T>(), with_w ? wbuf->as<T>() : nullptr, st);
+    PointSortBuffers<T> sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(),
+                           vals_alt.as<unsigned>(), hist.as<unsigned>(), pos.as<T>(),
+                           with_w ? wbuf->as<T>() : nullptr, pl.pos_alt.as<T>(), pl.pos_alt.as<T>() + (size_t)D * n};
+    if ((rc = sort_points<T, D>(sb, n, bp.bits, pts, w, with_w, st))) return rc;
+    return box_ranges(keys.as<unsigned>(), n, (unsigned)bp.NB, start.as<unsigned>(), st);
 }
 
 // The batched solve as a sequence of kernels on st (capturable: no
@@ -1537,7 +1535,7 @@ static std::vector<const void *> batch_buffer_ptrs(const Plan &pl, const void *t
             pl.s_w.p, pl.s_start.p, pl.t_keys.p, pl.t_vals.p, pl.t_keys_alt.p, pl.t_vals_alt.p, pl.t_hist.p,
             pl.t_pos.p, pl.t_start.p, pl.Wm.p, pl.Wl.p, pl.Wc.p, pl.Wc_lo.p, pl.Kf.p, pl.Kf_lo.p, pl.l2p_out.p,
             pl.p2p_slots.p, pl.d_nodes.p, pl.d_coef.p, pl.d_tab.p, pl.d_far.p, pl.d_cls_f.p, pl.d_cls_o.p,
-            pl.d_pairs.p,
+            pl.d_pairs.p, pl.pos_alt.p,
             b.d_toff.p, b.d_soff.p, b.d_levels.p, b.d_bbase.p, b.d_box_prob.p, b.d_geoms.p, b.d_items.p,
             b.d_tasks.p, b.d_ent_level.p, b.d_ent_prob.p, pl.ops.ctx};
 }

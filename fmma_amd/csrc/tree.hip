@@ -139,26 +139,12 @@ int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_si
 
 // ---------------------------------------------------------------- leaf keys
 
-// With rec != NULL the kernel also packs each point into one aligned record
-// (its D coordinates and its weight, kRecStride<D> values), so that the
-// gather after the sort reads one record per point instead of a coordinate
-// triple and a weight from two random places.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
-                                                   int grid_n, unsigned *keys, const T *w, T *rec)
+                                                   int grid_n, unsigned *keys)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    if (rec) {
-        T r[kRecStride<D>];
-#pragma unroll
-        for (int d = 0; d < D; ++d) r[d] = pts[i * D + d];
-        r[D] = w ? w[i] : T(1);
-#pragma unroll
-        for (int d = D + 1; d < kRecStride<D>; ++d) r[d] = T(0);
-#pragma unroll
-        for (int d = 0; d < kRecStride<D>; ++d) rec[i * kRecStride<D> + d] = r[d];
-    }
     const int NL = grid_n > 0 ? grid_n : 1 << leaf_level;
     const T len = g->len_leaf;
     int b[D];
@@ -175,12 +161,11 @@ __global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const G
 }
 
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st,
-              const T *w, T *rec)
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st)
 {
     if (n <= 0) return FMMA_OK;
     hipLaunchKernelGGL((keys_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, n, g,
-                       leaf_level, grid_n, keys, w, rec);
+                       leaf_level, grid_n, keys);
     FMMA_LAUNCHED("leaf_keys");
     return FMMA_OK;
 }
@@ -341,6 +326,123 @@ __global__ __launch_bounds__(256) void rs_scatter(const unsigned *kin, const uns
     }
 }
 
+// One pass of the point sort.  Every value of a thread's 16 points (key,
+// index, coordinates, weight) is loaded at once, so the loads overlap the
+// ranking.  A tile of 4096 points is ranked stably by the 8-bit digit at
+// `shift`: wave w ranks its contiguous quarter of the tile in 16 rounds of
+// 64 against counters of its own (ballots find the lanes of a round that
+// share a digit), so no barrier is needed until the four waves' counts are
+// combined.  Every array then goes through one LDS staging tile in sorted
+// order, and lane j writes staged element j to its digit's global run: the
+// writes of a run are consecutive lanes.  The first pass reads the caller's
+// points (AoS coordinates, weights or 1, the index as the value); later
+// passes read the previous pass's SoA output.
+template <typename T, int D, bool FIRST, bool WITH_W>
+__global__ __launch_bounds__(256) void rs_scatter_points(PointPass<T> a)
+{
+    constexpr int NV = WITH_W ? D + 1 : D;
+    __shared__ unsigned wcnt[4][256];
+    __shared__ unsigned loc[256], gdel[256], wsum[4];
+    __shared__ unsigned skey[RS_TILE];
+    __shared__ __align__(16) T stage[RS_TILE];
+    unsigned *stage_u = reinterpret_cast<unsigned *>(stage);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // Workgroups are dealt to the 8 XCDs round-robin; XCD x takes a
+    // contiguous run of tiles, so the runs of one digit that neighbouring
+    // tiles write side by side meet in one L2 and leave it as whole lines.
+    const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
+    const unsigned tile = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const long n = a.n, tile0 = (long)tile * RS_TILE;
+    const int cnt = (int)(n - tile0 < RS_TILE ? n - tile0 : RS_TILE);
+    const long my0 = tile0 + (long)wave * (RS_TILE / 4) + lane;
+
+    unsigned key[RS_ITEMS], idx[RS_ITEMS];
+    T val[NV][RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long i = my0 + r * 64;
+        const bool valid = i < n;
+        key[r] = valid ? a.kin[i] : 0u;
+        idx[r] = FIRST ? (unsigned)i : valid ? a.vin[i] : 0u;
+#pragma unroll
+        for (int d = 0; d < NV; ++d) {
+            if (!valid)
+                val[d][r] = T(0);
+            else if (d < D)
+                val[d][r] = FIRST ? a.pin[i * D + d] : a.pin[(long)d * n + i];
+            else
+                val[d][r] = FIRST ? (a.win ? a.win[i] : T(1)) : a.win[i];
+        }
+    }
+
+    for (int j = lane; j < 256; j += 64) wcnt[wave][j] = 0u;
+    const unsigned long long below = __lanemask_lt();
+    unsigned lp[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const bool valid = my0 + r * 64 < n;
+        const unsigned dig = (key[r] >> a.shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dig >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const unsigned rank = __popcll(peers & below);
+        const unsigned pre = wcnt[wave][dig];
+        if (valid && rank == 0) wcnt[wave][dig] = pre + (unsigned)__popcll(peers);
+        lp[r] = pre + rank;
+    }
+    __syncthreads();
+    {  // thread tid owns digit tid: wave prefixes, the tile-local start, the global start
+        const unsigned c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+        const unsigned tot = c0 + c1 + c2 + c3;
+        wcnt[0][tid] = 0u;
+        wcnt[1][tid] = c0;
+        wcnt[2][tid] = c0 + c1;
+        wcnt[3][tid] = c0 + c1 + c2;
+        unsigned all;
+        const unsigned start = block_scan256(tot, all, wsum) - tot;
+        loc[tid] = start;
+        gdel[tid] = a.offs[(long)tid * a.ntiles + tile] - start;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (my0 + r * 64 < n) {
+            const unsigned dig = (key[r] >> a.shift) & 255u;
+            lp[r] += loc[dig] + wcnt[wave][dig];
+            skey[lp[r]] = key[r];
+            stage_u[lp[r]] = idx[r];
+        }
+    }
+    __syncthreads();
+    unsigned gout[RS_ITEMS];  // global place of staged element r * 256 + tid
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int j = r * 256 + tid;
+        if (j < cnt) {
+            const unsigned k = skey[j];
+            gout[r] = gdel[(k >> a.shift) & 255u] + (unsigned)j;
+            a.kout[gout[r]] = k;
+            a.vout[gout[r]] = stage_u[j];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < NV; ++d) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RS_ITEMS; ++r)
+            if (my0 + r * 64 < n) stage[lp[r]] = val[d][r];
+        __syncthreads();
+        T *out = d < D ? a.pout + (long)d * n : a.wout;
+#pragma unroll
+        for (int r = 0; r < RS_ITEMS; ++r)
+            if (r * 256 + tid < cnt) out[gout[r]] = stage[r * 256 + tid];
+    }
+}
+
 __global__ __launch_bounds__(256) void iota_kernel(unsigned *v, long n)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -380,6 +482,55 @@ int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st)
     if (vin != b.vals) {
         FMMA_HIP(hipMemcpyAsync(b.vals, vin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
     }
+    return FMMA_OK;
+}
+
+template <typename T, int D>
+int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st)
+{
+    if (n <= 0) return FMMA_OK;
+    if (bits <= 0) {  // one box: the identity permutation
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b.vals, n);
+        FMMA_LAUNCHED("iota");
+        return gather<T, D>(pts, w, b.vals, n, b.pos, with_w ? b.ws : nullptr, st);
+    }
+    const unsigned ntiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
+    const int passes = (bits + 7) / 8;
+    // keys and indices ping-pong from (keys, vals) and end in the pair the
+    // last pass wrote; coordinates and weights start from the caller's
+    // arrays and alternate so that the last pass writes (pos, ws)
+    unsigned *kin = b.keys, *kout = b.keys_alt, *vin = nullptr, *vout = b.vals_alt;
+    const T *pin = pts, *win = w;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_final = (passes - 1 - p) % 2 == 0;
+        T *pout = to_final ? b.pos : b.pos_alt, *wout = to_final ? b.ws : b.ws_alt;
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
+        int rc = scan_exclusive(b.hist, (long)ntiles * 256, b.hist + radix_hist_words(n) - SCAN_BLOCKS, st);
+        if (rc) return rc;
+        PointPass<T> a{kin, vin, kout, vout, pin, win, pout, wout, n, shift, b.hist, ntiles};
+        if (p == 0) {
+            if (with_w)
+                hipLaunchKernelGGL((rs_scatter_points<T, D, true, true>), dim3(ntiles), dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL((rs_scatter_points<T, D, true, false>), dim3(ntiles), dim3(256), 0, st, a);
+        } else {
+            if (with_w)
+                hipLaunchKernelGGL((rs_scatter_points<T, D, false, true>), dim3(ntiles), dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL((rs_scatter_points<T, D, false, false>), dim3(ntiles), dim3(256), 0, st, a);
+        }
+        FMMA_LAUNCHED("sort_points");
+        unsigned *kprev = kin;
+        kin = kout;
+        vin = vout;
+        kout = kprev;
+        vout = (vout == b.vals_alt) ? b.vals : b.vals_alt;
+        pin = pout;
+        win = wout;
+    }
+    if (kin != b.keys) FMMA_HIP(hipMemcpyAsync(b.keys, kin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+    if (vin != b.vals) FMMA_HIP(hipMemcpyAsync(b.vals, vin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
     return FMMA_OK;
 }
 
@@ -423,32 +574,6 @@ __global__ __launch_bounds__(256) void gather_kernel(const T *pts, const T *w, c
     if (ws) ws[i] = w ? w[p] : T(1);
 }
 
-// The same from the packed records of keys_kernel: one aligned record per
-// point (16 or 32 bytes), read with vector loads.
-template <typename T, int D>
-__global__ __launch_bounds__(256) void gather_records_kernel(const T *rec, const unsigned *perm, long n, T *pos, T *ws)
-{
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const unsigned p = perm[i];
-    T r[kRecStride<D>];
-#pragma unroll
-    for (int d = 0; d < kRecStride<D>; ++d) r[d] = rec[(long)p * kRecStride<D> + d];
-#pragma unroll
-    for (int d = 0; d < D; ++d) pos[d * n + i] = r[d];
-    if (ws) ws[i] = r[D];
-}
-
-template <typename T, int D>
-int gather_records(const T *rec, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st)
-{
-    if (n <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((gather_records_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rec, perm,
-                       n, pos, ws);
-    FMMA_LAUNCHED("gather_records");
-    return FMMA_OK;
-}
-
 template <typename T, int D>
 int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st)
 {
@@ -461,10 +586,9 @@ int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws
 
 #define FMMA_TREE_INST(T, D)                                                                                    \
     template int bbox<T, D>(const T *, long, const T *, long, T *, Geom<T> *, int, hipStream_t);                \
-    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t, const T *,  \
-                                 T *);                                                                         \
+    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t);        \
     template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);              \
-    template int gather_records<T, D>(const T *, const unsigned *, long, T *, T *, hipStream_t);
+    template int sort_points<T, D>(PointSortBuffers<T> &, long, int, const T *, const T *, bool, hipStream_t);
 FMMA_TREE_INST(double, 1)
 FMMA_TREE_INST(double, 2)
 FMMA_TREE_INST(double, 3)

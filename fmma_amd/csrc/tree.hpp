@@ -17,35 +17,54 @@ struct SortBuffers {
     unsigned *hist;      // [radix_hist_words(n)]
 };
 
+// The point sort: keys and indices as in SortBuffers, plus the sorted
+// coordinates (SoA, D x n) and weights, and SoA buffers of the same sizes
+// for the passes in between (ws, ws_alt unused without weights).
+template <typename T>
+struct PointSortBuffers {
+    unsigned *keys, *vals, *keys_alt, *vals_alt, *hist;
+    T *pos, *ws, *pos_alt, *ws_alt;
+};
+
+// Arguments of one pass of the point sort (tree.hip).
+template <typename T>
+struct PointPass {
+    const unsigned *kin, *vin;  // vin unused in the first pass (the index is the value)
+    unsigned *kout, *vout;
+    const T *pin, *win;  // first pass: AoS coordinates, weights (NULL: 1); then SoA
+    T *pout, *wout;
+    long n;
+    int shift;
+    const unsigned *offs;  // scattered digit starts, [256][ntiles]
+    unsigned ntiles;
+};
+
 // The bounding cube of targets and sources; len_leaf = Len / n_side.
 template <typename T, int D>
 int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_side, hipStream_t st);
 
-// Values per packed point record: D coordinates and the weight, padded to
-// 16 bytes (D = 1) or 32 bytes (D = 2, 3) for double.
-template <int D>
-constexpr int kRecStride = D == 1 ? 2 : 4;
-
 // Leaf box of each point: Morton key on the 2^leaf_level grid, or with
-// grid_n > 0 the row-major index on a grid_n^D grid.  With rec != NULL also
-// each point's packed record (coordinates, then w[i], or 1 without w).
+// grid_n > 0 the row-major index on a grid_n^D grid.
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st,
-              const T *w = nullptr, T *rec = nullptr);
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st);
 
 // Stable sort of b.keys (low `bits` bits significant); b.vals gets the
 // permutation.  bits == 0 only fills b.vals with 0..n-1.
 int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st);
 size_t radix_hist_words(long n);
 
+// Stable sort of the points (pts: n x D AoS; w: weights, NULL for 1) by
+// b.keys, the low `bits` bits significant: b.keys sorted, b.vals the
+// original index of each sorted point, b.pos its coordinates (SoA), b.ws its
+// weight when with_w.  The coordinates and weights move with the keys in
+// every pass, so no gather by permutation follows.
+template <typename T, int D>
+int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st);
+
 int box_ranges(const unsigned *keys, long n, unsigned nbox, unsigned *start, hipStream_t st);
 
 template <typename T, int D>
 int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st);
-
-// pos[d*n + i] = rec[perm[i]] coordinates, ws[i] (if not NULL) its weight.
-template <typename T, int D>
-int gather_records(const T *rec, const unsigned *perm, long n, T *pos, T *ws, hipStream_t st);
 
 }  // namespace lib
 }  // namespace fmma
