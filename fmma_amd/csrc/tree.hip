@@ -172,11 +172,10 @@ int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n
 
 // --------------------------------------------------------------- radix sort
 //
-// 8-bit digits, tiles of 4096 keys.  rs_hist counts digits per tile,
-// scan_exclusive turns the digit-major [256][tiles] table into scatter bases,
-// and rs_scatter ranks keys stably inside the tile: lanes of one wave that
-// share a digit are found with 8 ballots, waves are ordered through LDS
-// counts, and 16 rounds of 256 keys keep the original order across the tile.
+// LSD passes of 8-bit digits over tiles of 4096 keys.  rs_hist counts digits
+// per tile, scan_exclusive turns the digit-major [256][tiles] table into
+// scatter bases, and rs_scatter_points ranks the tile's keys stably and moves
+// every array of the points to its sorted place.
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
@@ -280,50 +279,6 @@ static int scan_exclusive(unsigned *a, long m, unsigned *part, hipStream_t st)
     hipLaunchKernelGGL(scan_down, dim3(nb), dim3(256), 0, st, a, m, chunk, part);
     FMMA_LAUNCHED("scan_exclusive");
     return FMMA_OK;
-}
-
-__global__ __launch_bounds__(256) void rs_scatter(const unsigned *kin, const unsigned *vin, unsigned *kout,
-                                                  unsigned *vout, long n, int shift, const unsigned *offs,
-                                                  unsigned ntiles)
-{
-    __shared__ unsigned base_s[256];
-    __shared__ unsigned run_s[256];
-    __shared__ unsigned wc[4][256];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    base_s[tid] = offs[(long)tid * ntiles + blockIdx.x];
-    run_s[tid] = 0;
-    wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
-    __syncthreads();
-    const unsigned long long below = __lanemask_lt();
-    const long tile0 = (long)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        const long i = tile0 + (long)r * RS_THREADS + tid;
-        const bool valid = i < n;
-        const unsigned key = valid ? kin[i] : 0u;
-        const unsigned val = valid ? (vin ? vin[i] : (unsigned)i) : 0u;
-        const unsigned dig = (key >> shift) & 255u;
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (dig >> b) & 1u;
-            const unsigned long long bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
-        const unsigned rank = __popcll(peers & below);
-        if (valid && rank == 0) wc[wave][dig] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            unsigned pre = run_s[dig];
-            for (int w = 0; w < wave; ++w) pre += wc[w][dig];
-            const unsigned dst = base_s[dig] + pre + rank;
-            kout[dst] = key;
-            vout[dst] = val;
-        }
-        __syncthreads();
-        run_s[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
-        wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
-        __syncthreads();
-    }
 }
 
 // One pass of the point sort.  Every value of a thread's 16 points (key,
@@ -447,42 +402,6 @@ __global__ __launch_bounds__(256) void iota_kernel(unsigned *v, long n)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i < n) v[i] = (unsigned)i;
-}
-
-int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st)
-{
-    if (n <= 0) return FMMA_OK;
-    if (bits <= 0) {
-        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b.vals, n);
-        FMMA_LAUNCHED("iota");
-        return FMMA_OK;
-    }
-    const unsigned ntiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
-    const int passes = (bits + 7) / 8;
-    unsigned *kin = b.keys, *kout = b.keys_alt, *vin = nullptr, *vout = b.vals_alt;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
-        int rc = scan_exclusive(b.hist, (long)ntiles * 256, b.hist + radix_hist_words(n) - SCAN_BLOCKS, st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(rs_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, n, shift, b.hist,
-                           ntiles);
-        FMMA_LAUNCHED("radix_sort");
-        // ping-pong; after the pass the sorted data is in (kout, vout)
-        unsigned *kprev = kin;
-        kin = kout;
-        vin = vout;
-        kout = kprev;
-        vout = (vout == b.vals_alt) ? b.vals : b.vals_alt;
-    }
-    // make (b.keys, b.vals) the sorted result
-    if (kin != b.keys) {
-        FMMA_HIP(hipMemcpyAsync(b.keys, kin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
-    }
-    if (vin != b.vals) {
-        FMMA_HIP(hipMemcpyAsync(b.vals, vin, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
-    }
-    return FMMA_OK;
 }
 
 template <typename T, int D>

@@ -9,17 +9,11 @@ constexpr int kBBoxParts = 1024;  // partial min/max blocks of the bounding-cube
 template <int D>
 constexpr int kBBoxStride = 2 * D + 1;  // per block: min[D], max[D], non-finite flag
 
-struct SortBuffers {
-    unsigned *keys;      // [n] in: leaf keys; out: sorted keys
-    unsigned *vals;      // [n] out: original index of each sorted point
-    unsigned *keys_alt;  // [n] ping-pong
-    unsigned *vals_alt;  // [n]
-    unsigned *hist;      // [radix_hist_words(n)]
-};
-
-// The point sort: keys and indices as in SortBuffers, plus the sorted
-// coordinates (SoA, D x n) and weights, and SoA buffers of the same sizes
-// for the passes in between (ws, ws_alt unused without weights).
+// The point sort: keys[n] (in: leaf keys; out: sorted), vals[n] (out: the
+// original index of each sorted point), keys_alt/vals_alt (ping-pong),
+// hist[radix_hist_words(n)], the sorted coordinates (SoA, D x n) and
+// weights, and SoA buffers of the same sizes for the passes in between (ws,
+// ws_alt unused without weights).
 template <typename T>
 struct PointSortBuffers {
     unsigned *keys, *vals, *keys_alt, *vals_alt, *hist;
@@ -48,9 +42,6 @@ int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_si
 template <typename T, int D>
 int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st);
 
-// Stable sort of b.keys (low `bits` bits significant); b.vals gets the
-// permutation.  bits == 0 only fills b.vals with 0..n-1.
-int radix_sort(SortBuffers &b, long n, int bits, hipStream_t st);
 size_t radix_hist_words(long n);
 
 // Stable sort of the points (pts: n x D AoS; w: weights, NULL for 1) by
