@@ -278,8 +278,12 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
     constexpr int NT = ipow_c(NN, D - 1);  // trailing index tuples
     constexpr int G = NT <= 64 ? 64 / NT : 1;
     static_assert(NT <= 64, "at most 64 trailing index tuples (8 nodes in 3-D)");
+    // rows of an odd number of values: the 64 lanes writing their rows hit
+    // different banks (an even stride of 16 doubles put every lane on the
+    // same two banks)
+    constexpr int SV = (D * NN) | 1;
     __shared__ T sc[NN * NN];
-    __shared__ T sv[64][D * NN + 1];
+    __shared__ T sv[64][SV];
     __shared__ T red[G > 1 ? G - 1 : 1][NT * NN];
     const unsigned box = blockIdx.x;
     const int lane = threadIdx.x;

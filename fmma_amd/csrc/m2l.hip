@@ -144,7 +144,8 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
     }
     n_cls -= n3;  // interaction offsets per class
     const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
-    const long gy = (MP / 16 + 4 * rb - 1) / (4 * rb);  // workgroups per task (row chunks)
+    // workgroups per task (row chunks); one for the 32x32x2 form (4 waves x 32 rows)
+    const long gy = m2l_wide(mode, KP) ? 1 : (MP / 16 + 4 * rb - 1) / (4 * rb);
     // Task order inside a level: tile-major (a tile's 2^D classes together)
     // or class-major (all tiles of one class together, so that each XCD's
     // contiguous share holds few classes and its tasks read the same
@@ -213,6 +214,15 @@ long m2l_split_elems(const std::vector<M2LSplit> &splits)
     return e;
 }
 
+bool m2l_wide(int mode, int KP) { return mode == 1 && KP == 128; }
+
+int m2l_frag_format(int P, int mode)
+{
+    int MP, KP;
+    m2l_shape(P, mode, MP, KP);
+    return mode == 0 ? FMMA_FRAG_F64 : mode == 2 ? FMMA_FRAG_BF16 : m2l_wide(mode, KP) ? FMMA_FRAG_F32W : FMMA_FRAG_F32;
+}
+
 void m2l_shape(int P, int mode, int &MP, int &KP)
 {
     MP = (P + 15) / 16 * 16;
@@ -236,6 +246,7 @@ void m2l_shape(int P, int mode, int &MP, int &KP)
     } else {
         KP = (P + 31) / 32 * 32;
     }
+    if (m2l_wide(mode, KP)) MP = 128;  // whole 32-row blocks
 }
 
 size_t m2l_frag_elems(int n_far, int P, int mode)
@@ -248,14 +259,15 @@ size_t m2l_frag_elems(int n_far, int P, int mode)
 // --------------------------------------------------------------- fragments
 
 template <typename T, typename CT>
-__global__ __launch_bounds__(256) void frag4_kernel(const T *Kraw, CT *Kf, int P, int MP, int KP, long total)
+__global__ __launch_bounds__(256) void frag4_kernel(const T *Kraw, CT *Kf, int P, int MP, int KP, long total,
+                                                    int format)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) return;
     const long per = (long)MP * KP;
     const long f = e / per;
     int row, k;
-    detail::frag_rowcol(FMMA_FRAG_F32, KP, e - f * per, row, k);
+    detail::frag_rowcol(format, KP, e - f * per, row, k);
     Kf[e] = (row < P && k < P) ? (CT)Kraw[(f * P + row) * P + k] : CT(0);
 }
 
@@ -283,9 +295,11 @@ int m2l_fragments(const T *Kraw, int n_far, int P, int mode, void *Kf, void *Kf_
     const long total = (long)n_far * MP * KP;
     const unsigned nb = (unsigned)((total + 255) / 256);
     if (mode == 0)
-        hipLaunchKernelGGL((frag4_kernel<T, double>), dim3(nb), dim3(256), 0, st, Kraw, (double *)Kf, P, MP, KP, total);
+        hipLaunchKernelGGL((frag4_kernel<T, double>), dim3(nb), dim3(256), 0, st, Kraw, (double *)Kf, P, MP, KP, total,
+                           (int)FMMA_FRAG_F64);
     else if (mode == 1)
-        hipLaunchKernelGGL((frag4_kernel<T, float>), dim3(nb), dim3(256), 0, st, Kraw, (float *)Kf, P, MP, KP, total);
+        hipLaunchKernelGGL((frag4_kernel<T, float>), dim3(nb), dim3(256), 0, st, Kraw, (float *)Kf, P, MP, KP, total,
+                           m2l_frag_format(P, mode));
     else
         hipLaunchKernelGGL((frag_bf16_kernel<T>), dim3(nb), dim3(256), 0, st, Kraw, (__bf16 *)Kf, (__bf16 *)Kf_lo, P,
                            MP, KP, total);
@@ -830,8 +844,166 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
     return false;
 }
 
+// FP32 form at a padded length of 128 on the 32x32x2 MFMA: a workgroup of
+// 4 waves covers all 128 rows (wave w: rows 32 w .. 32 w + 31) of TB target
+// boxes, so a partner tile is gathered once per task and every operator
+// value and every staged operand value feeds twice the flops of the 16x16x4
+// shape.  The contraction runs in two chunks of 64 per offset, pipelined as
+// in m2l_pipe (gather and operator loads of step j + 1 during the MFMAs of
+// step j, two LDS tiles); an offset's FP32 sum is added into FP64 registers.
+// Operator fragments are FMMA_FRAG_F32W: one 16-byte load per lane brings 4
+// steps.
+template <typename TW, int D, int TB>
+__global__ __launch_bounds__(256) void m2l_pipe_wide(M2LKArgs a)
+{
+    constexpr int KP = 128, KC = 64, NCH = KP / KC;
+    constexpr int NG = KC / 8;                    // 16-byte operator groups per chunk (4 steps of 2)
+    constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 32;
+    constexpr int LDB = KC + 1;                   // odd: a 32-lane read of one k hits 32 banks
+    typedef float f16v __attribute__((ext_vector_type(16)));
+    static_assert(EPT % 4 == 0, "gather chunks must be whole 16-byte vectors");
+    constexpr int BUF = TB * LDB;
+    __shared__ __align__(16) float Bs[2 * BUF];
+
+    const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
+    const unsigned tix = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const M2LTask task = a.tasks[tix];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int N = 1 << task.level, P = a.P;
+    const int col = tid / TPC, part = tid % TPC;
+
+    int tc[D];
+    const bool cvalid = col < task.np;
+    {
+        int pc[D];
+        morton_decode<D>(task.p0 + (cvalid ? col : 0), pc);
+#pragma unroll
+        for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((task.cls >> (D - 1 - d)) & 1);
+    }
+    const float *Wg = static_cast<const float *>(a.Wc) + task.c_off;
+    const int i0 = task.i1 ? task.i0 : 0, i1 = task.i1 ? task.i1 : a.n_cls;
+    const int *cls_f = a.cls_f + task.cls * a.n_cls + i0;
+    const int *cls_o = a.cls_o + (task.cls * a.n_cls + i0) * D;
+    const float4 *Kf = reinterpret_cast<const float4 *>(static_cast<const float *>(a.Kf) + task.k_off);
+
+    float4 g[EPT / 4];
+    auto gather = [&](int j) {
+        const int i = j / NCH, c = j - i * NCH;
+        int m[D];
+        bool ok = cvalid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m[d] = tc[d] + cls_o[i * D + d];
+            ok = ok && m[d] >= 0 && m[d] < N;
+        }
+        const size_t off = (size_t)(ok ? morton_encode<D>(m) : 0u) * KP + c * KC + part * EPT;
+        const float4 *src = reinterpret_cast<const float4 *>(Wg + off);
+#pragma unroll
+        for (int e = 0; e < EPT / 4; ++e) g[e] = ok ? src[e] : float4{0, 0, 0, 0};
+    };
+    float4 afr[NG];
+    auto load_a = [&](int j) {
+        const int i = j / NCH, c = j - i * NCH;
+        const size_t base = (((size_t)cls_f[i] * 4 + wave) * (KP / 8) + (size_t)c * NG) * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < NG; ++q) afr[q] = Kf[base + (size_t)q * 64];
+    };
+    auto store_tile = [&](float *tile) {
+        float *dst = tile + col * LDB + part * EPT;
+#pragma unroll
+        for (int e = 0; e < EPT / 4; ++e) {
+            dst[4 * e] = g[e].x;
+            dst[4 * e + 1] = g[e].y;
+            dst[4 * e + 2] = g[e].z;
+            dst[4 * e + 3] = g[e].w;
+        }
+    };
+
+    f16v acc[NCB];
+    double accd[NCB][16];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            acc[cb][r] = 0.0f;
+            accd[cb][r] = 0.0;
+        }
+    }
+    const int n = (i1 - i0) * NCH;
+    gather(0);
+    load_a(0);
+    store_tile(Bs);
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        const float *cur = Bs + (i & 1) * BUF;
+        float4 acur[NG];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) acur[q] = afr[q];
+        if (i + 1 < n) {
+            gather(i + 1);
+            load_a(i + 1);
+        }
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float av = j == 0 ? acur[q].x : j == 1 ? acur[q].y : j == 2 ? acur[q].z : acur[q].w;
+                const int kk = 2 * (4 * q + j) + (lane >> 5);
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const float bv = cur[(32 * cb + (lane & 31)) * LDB + kk];
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[cb], 0, 0, 0);
+                }
+            }
+        }
+        if ((i + 1) % NCH == 0) {  // one offset summed: into FP64
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    accd[cb][r] += (double)acc[cb][r];
+                    acc[cb][r] = 0.0f;
+                }
+        }
+        if (i + 1 < n) {
+            store_tile(Bs + ((i + 1) & 1) * BUF);
+            __syncthreads();
+        }
+    }
+    TW *Wl = task.i1 ? static_cast<TW *>(a.Wpart) + task.o_off : static_cast<TW *>(a.Wl) + task.w_off;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+        const int c = 32 * cb + (lane & 31);
+        if (c >= task.np) continue;
+        const size_t box = ((size_t)(task.p0 + c) << D) | (unsigned)task.cls;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 32 * wave + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (row < P) Wl[box * P + row] = (TW)accd[cb][r];
+        }
+    }
+}
+
+template <typename T, int D>
+static bool launch_pipe_wide(const M2LKArgs &a, int tb, long ntasks, hipStream_t st)
+{
+    if constexpr (D == 1) {
+        return false;
+    } else {
+        const dim3 grid((unsigned)ntasks);
+        if (tb == 32)
+            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 32>), grid, dim3(256), 0, st, a);
+        else if (tb == 64)
+            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 64>), grid, dim3(256), 0, st, a);
+        else
+            return false;
+        return true;
+    }
+}
+
 bool m2l_pipe_supported(int D, int KP, int mode, int tb)
 {
+    if (m2l_wide(mode, KP)) return D >= 2 && (tb == 32 || tb == 64);
     const int vec = mode == 2 ? 8 : mode == 1 ? 4 : 2;
     const bool small = D >= 2 && (KP == 32 || KP == 64 || KP == 128 || (D == 2 && mode != 2 && (KP == 48 || KP == 96)));
     const bool big = D == 3 && (mode == 2 ? (KP == 256 || KP == 384 || KP == 512)
@@ -866,6 +1038,11 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
                 return FMMA_ERR_INVALID;
             }
         } else if (mode == 1) {
+            if (m2l_wide(mode, a.KP)) {  // the operators are in the 32x32x2 order: no other kernel reads them
+                if (!g.pipelined || !launch_pipe_wide<T, D>(a, g.tb, g.count, st)) return FMMA_ERR_UNSUPPORTED;
+                FMMA_LAUNCHED("m2l_apply");
+                continue;
+            }
             done = g.pipelined && launch_pipe<T, D, 1>(a, g.tb, grid, st);
             if (!done) {
                 const size_t lds =

@@ -91,6 +91,10 @@ struct M2LKArgs {
 
 // Shape of the operator fragments for a precision: rows MP, contraction KP.
 void m2l_shape(int P, int precision, int &MP, int &KP);
+// The FP32 form at a padded length of 128 runs on the 32x32x2 MFMA
+// (fragments FMMA_FRAG_F32W, MP = 128); the fragment format of a form.
+bool m2l_wide(int mode, int KP);
+int m2l_frag_format(int P, int mode);
 size_t m2l_frag_elems(int n_far, int P, int precision);  // elements per level (per part for bf16)
 
 // Kraw [n_far][P][P] (TYPE) -> fragments of level l.

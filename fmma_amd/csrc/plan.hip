@@ -278,11 +278,6 @@ static long tune_key(int D, int P, int mode, long npar)
     return (((long)D * 1024 + P) * 4 + mode) * (1L << 32) + npar;
 }
 
-static int frag_format(int mode)
-{
-    return mode == 0 ? FMMA_FRAG_F64 : mode == 1 ? FMMA_FRAG_F32 : FMMA_FRAG_BF16;
-}
-
 // The tile width of each level's tasks (by parents per level), as reported
 // by fmma_plan_m2l_tiles for single solves.
 static void record_tiles(Plan &pl, const std::vector<M2LTask> &tasks, int D)
@@ -618,7 +613,7 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
             oa.Len = &pl.geom.as<Geom<T>>()->Len;
             oa.offsets = pl.d_far.as<int>();
             oa.n_off = pl.offs.n_far;
-            oa.format = frag_format(mode);
+            oa.format = m2l_frag_format(P, mode);
             m2l_shape(P, mode, oa.MP, oa.KP);
             oa.n_batch = L - 1;
             oa.len_stride = 0;
@@ -1411,7 +1406,7 @@ static int batch_kernels(Plan &pl, BatchPlan &bp, const T *tg, const T *w, const
     oa.len_stride = (int)(sizeof(Geom<T>) / sizeof(T));
     oa.offsets = pl.d_far.as<int>();
     oa.n_off = pl.offs.n_far;
-    oa.format = frag_format(mode);
+    oa.format = m2l_frag_format(P, mode);
     m2l_shape(P, mode, oa.MP, oa.KP);
     oa.n_batch = (int)bp.ent_level.size();
     oa.batch_level = bp.d_ent_level.as<int>();

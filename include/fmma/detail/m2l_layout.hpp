@@ -7,6 +7,8 @@
 //       r = (rb * (KP/4) + s) * 64 + lane,      row = 16 rb + lane % 16, col = 4 s + lane / 16
 //   FMMA_FRAG_BF16 (16x16x32, hi and lo parts in two arrays):
 //       r = ((rb * (KP/32) + s) * 64 + lane) * 8 + j,  row = 16 rb + lane % 16, col = 32 s + 8 (lane / 16) + j
+//   FMMA_FRAG_F32W (32x32x2, four steps per lane side by side for one 16-byte load; MP = 128):
+//       r = ((rb * (KP/8) + g) * 64 + lane) * 4 + j,   row = 32 rb + lane % 32, col = 2 (4 g + j) + lane / 32
 #pragma once
 #include "../../fmma_hip.h"
 
@@ -15,7 +17,13 @@ namespace detail {
 
 __host__ __device__ inline void frag_rowcol(int format, int KP, long r, int &row, int &col)
 {
-    if (format == FMMA_FRAG_BF16) {
+    if (format == FMMA_FRAG_F32W) {
+        const int j = (int)(r & 3), lane = (int)((r >> 2) & 63);
+        const long q = r >> 8;
+        const int g = (int)(q % (KP / 8)), rb = (int)(q / (KP / 8));
+        row = 32 * rb + (lane & 31);
+        col = 2 * (4 * g + j) + (lane >> 5);
+    } else if (format == FMMA_FRAG_BF16) {
         const int j = (int)(r & 7), lane = (int)((r >> 3) & 63);
         const long q = r >> 9;
         const int s = (int)(q % (KP / 32)), rb = (int)(q / (KP / 32));

@@ -1099,6 +1099,10 @@ __global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
 // Fragment index of element (row, col) of one offset's block.
 __device__ __forceinline__ long frag_index(int format, int KP, int row, int col)
 {
+    if (format == FMMA_FRAG_F32W) {
+        const int lane = (row & 31) + 32 * (col & 1), s = col >> 1;
+        return ((((long)(row >> 5) * (KP / 8) + (s >> 2)) * 64 + lane) << 2) + (s & 3);
+    }
     if (format == FMMA_FRAG_BF16) {
         const int lane = (row & 15) + 16 * ((col & 31) >> 3);
         return ((((long)(row >> 4) * (KP / 32) + (col >> 5)) * 64 + lane) << 3) + (col & 7);
@@ -1110,8 +1114,8 @@ __device__ __forceinline__ long frag_index(int format, int KP, int row, int col)
 // Operator sets of several (problem, level) pairs straight into fragment
 // order, with the same values and roundings as m2l_op_kernel followed by the
 // library's fragment conversion.  Each wave covers 64 consecutive fragment
-// elements: one 16-row block of K_o and 4 columns (16x16x4 formats) or 8
-// (BF16).  Their node positions X (rows) and Y (columns) are computed once
+// elements: 16 rows of K_o and 4 columns (16x16x4 formats: columns 4 s ..
+// 4 s + 3; F32W: every second column from 8 g + lane / 32) or 8 (BF16).  Their node positions X (rows) and Y (columns) are computed once
 // per wave into LDS with exactly m2l_op_value's expressions; each thread then
 // evaluates one f(X, Y).  PAIRED: blockIdx.y is a pair (o, -o) and
 // K_{-o}[col][row] = sym * K_o[row][col] is stored from the same value
@@ -1123,18 +1127,30 @@ __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K 
     const int b = blockIdx.z, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oi = PAIRED ? a.pairs[2 * blockIdx.y] : (int)blockIdx.y;
     const long per = (long)a.MP * a.KP;
-    const bool bf = a.format == FMMA_FRAG_BF16;
-    const int nks = bf ? a.KP / 32 : a.KP / 4;
+    const bool bf = a.format == FMMA_FRAG_BF16, wide = a.format == FMMA_FRAG_F32W;
     const long rw = (long)blockIdx.x * 256 + wave * 64;  // the wave's first element
-    const long q = bf ? (rw >> 9) : (rw >> 6);
-    const int s = (int)(q % nks), rb = (int)(q / nks);
-    const int cbase = bf ? 32 * s + 8 * ((int)((rw >> 3) & 63) >> 4) : 4 * s;
+    int rbase, cbase, cstep;                             // rows rbase + i, columns cbase + cstep j
+    if (wide) {
+        const int l0 = (int)((rw >> 2) & 63);
+        const long q = rw >> 8;
+        const int g = (int)(q % (a.KP / 8)), rb = (int)(q / (a.KP / 8));
+        rbase = 32 * rb + (l0 & 31);
+        cbase = 8 * g + (l0 >> 5);
+        cstep = 2;
+    } else {
+        const int nks = bf ? a.KP / 32 : a.KP / 4;
+        const long q = bf ? (rw >> 9) : (rw >> 6);
+        const int s = (int)(q % nks), rb = (int)(q / nks);
+        rbase = 16 * rb;
+        cbase = bf ? 32 * s + 8 * ((int)((rw >> 3) & 63) >> 4) : 4 * s;
+        cstep = 1;
+    }
     const int ncol = bf ? 8 : 4;
     const T Len = static_cast<const T *>(a.Len)[(long)a.batch_problem[b] * a.len_stride];
     const T len = Len / (T)(1u << a.batch_level[b]);
     if (lane < 16 + ncol) {
         const bool isrow = lane < 16;
-        const int k = isrow ? 16 * rb + lane : cbase + lane - 16;
+        const int k = isrow ? rbase + lane : cbase + cstep * (lane - 16);
         if (k < a.P) {
             int dk = k;
 #pragma unroll
@@ -1155,12 +1171,12 @@ __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K 
     frag_rowcol(a.format, a.KP, r, row, col);
     const bool in = row < a.P && col < a.P;
     T v = T(0);
-    if (in) v = kern.template operator()<T, D>(sx[wave][row - 16 * rb], sy[wave][col - cbase]);
+    if (in) v = kern.template operator()<T, D>(sx[wave][row - rbase], sy[wave][(col - cbase) / cstep]);
     auto store = [&](int off, long e, T x) {
         const long g = ((long)b * a.n_off + off) * per + e;
         if (a.format == FMMA_FRAG_F64) {
             static_cast<double *>(a.Kf)[g] = (double)x;
-        } else if (a.format == FMMA_FRAG_F32) {
+        } else if (a.format == FMMA_FRAG_F32 || a.format == FMMA_FRAG_F32W) {
             static_cast<float *>(a.Kf)[g] = (float)x;
         } else {
             const float fv = (float)x;
