@@ -350,6 +350,82 @@ __global__ __launch_bounds__(64) void p2m_ct(P2MArgs<T> a)
         for (int k = 0; k < NN; ++k) a.Wm[(size_t)box * P + k * NT + c] = acc[k];
 }
 
+// P2M in 3-D for NN <= 6 with fewer LDS reads per multiply-add: lane (g, a2)
+// keeps all NN^2 sums (a0, a1) of its last index a2 in registers and takes
+// every G-th point of a staged chunk (G = 64 / NN groups): per point it reads
+// the point's NN weights of dimensions 0 and 1 (the same addresses for the
+// lanes of a group) and one weight of dimension 2, forms S1[a1] S2[a2] and
+// adds S0[a0] times that to the NN^2 sums.  The groups are added at the end
+// in a fixed order through LDS.
+template <typename T, int NN>
+__global__ __launch_bounds__(64) void p2m3_ct(P2MArgs<T> a)
+{
+    constexpr int D = 3, P = NN * NN * NN, G = 64 / NN, SV = (D * NN) | 1;
+    constexpr int NS = 64 * SV > G * P ? 64 * SV : G * P;
+    __shared__ T sc[NN * NN];
+    __shared__ T smem[NS];  // the staged weights, then the groups' sums
+    T(*sv)[SV] = reinterpret_cast<T(*)[SV]>(smem);
+    T(*red)[P] = reinterpret_cast<T(*)[P]>(smem);
+    const unsigned box = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < NN * NN; i += 64) sc[i] = a.coef[i];
+    const unsigned s0 = a.start[box], s1 = a.start[box + 1];
+    const int g = lane / NN, a2 = lane - g * NN;
+    const bool act = g < G;
+    T acc[NN][NN];
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+        for (int j = 0; j < NN; ++j) acc[i][j] = T(0);
+    const Geom<T> *gm;
+    const int lv = box_geom<T>(a, box, gm), NL = a.grid_n > 0 ? a.grid_n : 1 << lv;
+    const T len = gm->len_leaf;
+    __syncthreads();
+    for (unsigned base = s0; base < s1; base += 64) {
+        const unsigned i = base + lane;
+        if (i < s1) {
+            const T wt = a.w[i];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                T rel;
+                leaf_coord_rel(a.pos[d * a.n + i], gm->origin[d], len, NL, rel);
+                T out[NN];
+                cheb_weights_ct<T, NN>(sc, rel, d == 0 ? wt : T(1), out);
+#pragma unroll
+                for (int k = 0; k < NN; ++k) sv[lane][d * NN + k] = out[k];
+            }
+        }
+        __syncthreads();
+        const int cnt = (int)(s1 - base < 64u ? s1 - base : 64u);
+        if (act) {
+            for (int p = g; p < cnt; p += G) {
+                const T w2 = sv[p][2 * NN + a2];
+                T t1[NN];
+#pragma unroll
+                for (int j = 0; j < NN; ++j) t1[j] = sv[p][NN + j] * w2;
+#pragma unroll
+                for (int k = 0; k < NN; ++k) {
+                    const T w0 = sv[p][k];
+#pragma unroll
+                    for (int j = 0; j < NN; ++j) acc[k][j] += w0 * t1[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (act)
+#pragma unroll
+        for (int k = 0; k < NN; ++k)
+#pragma unroll
+            for (int j = 0; j < NN; ++j) red[g][(k * NN + j) * NN + a2] = acc[k][j];
+    __syncthreads();
+    for (int e = lane; e < P; e += 64) {
+        T v = red[0][e];
+        for (int h = 1; h < G; ++h) v += red[h][e];
+        a.Wm[(size_t)box * P + e] = v;
+    }
+}
+
 // out[..k_d..] = sum_c M[k_d][c] in[..c..] along dimension d of an NN^D
 // tensor (dimension 0 most significant); M is NN x NN (row = output index).
 template <typename T, int D, int NN>
@@ -459,6 +535,106 @@ __global__ __launch_bounds__(64) void l2l_ct(const T *Wp, T *Wc, const T *tab, c
             }
         }
         __syncthreads();
+    }
+}
+
+// M2M and L2L with one wave per child (2^D waves per parent): the same
+// transfers as m2m_ct / l2l_ct, in the same order of operations, so the
+// results are bit-identical, but the children's passes run side by side
+// instead of one after another (a short serial chain per parent, which is
+// what the few-parent upper levels are bound by).  M2M adds the children's
+// contributions in child order through LDS.
+template <int D, int NN>
+constexpr bool kWideTransfer = ipow_c(NN, D) * (3 << D) * 8 <= 48 * 1024;
+
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(64 << D) void m2m_w(const T *Wc, T *Wp, const T *tab, const long *items)
+{
+    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
+    __shared__ T st[2 * NN * NN];
+    __shared__ T sw[NC * P];  // the children's multipoles, then their contributions
+    __shared__ T tA[NC][P], tB[NC][P];
+    const unsigned parent = blockIdx.x;
+    const int tid = threadIdx.x, sg = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 2 * NN * NN; i += 64 * NC) st[i] = tab[i];
+    const T *src = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * NC * P);
+    for (int i = tid; i < NC * P; i += 64 * NC) sw[i] = src[i];
+    __syncthreads();
+    const T *cur = sw + sg * P;
+    T *bufs[2] = {tA[sg], tB[sg]};
+    int nb = 0;
+    for (int d = D - 1; d >= 1; --d) {
+        const int bit = (sg >> (D - 1 - d)) & 1;
+        apply_dim<T, D, NN>(st + bit * NN * NN, false, d, cur, bufs[nb], lane);
+        __syncthreads();
+        cur = bufs[nb];
+        nb ^= 1;
+    }
+    const T *M = st + ((sg >> (D - 1)) & 1) * NN * NN;
+    constexpr int stride = ipow_c(NN, D - 1);
+    T v[KPL];
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const int e = lane + 64 * r;
+        v[r] = T(0);
+        if (e < P) {
+            const int k0 = e / stride, base = e - k0 * stride;
+#pragma unroll
+            for (int c = 0; c < NN; ++c) v[r] += M[k0 * NN + c] * cur[base + c * stride];
+        }
+    }
+    __syncthreads();  // every wave is done with sw
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const int e = lane + 64 * r;
+        if (e < P) sw[sg * P + e] = v[r];
+    }
+    __syncthreads();
+    T *dst = Wp + (items ? (size_t)items[2 * parent + 1] : (size_t)parent * P);
+    for (int e = tid; e < P; e += 64 * NC) {
+        T acc = T(0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc += sw[c * P + e];
+        dst[e] = acc;
+    }
+}
+
+template <typename T, int D, int NN>
+__global__ __launch_bounds__(64 << D) void l2l_w(const T *Wp, T *Wc, const T *tab, const long *items)
+{
+    constexpr int P = ipow_c(NN, D), KPL = (P + 63) / 64, NC = 1 << D;
+    __shared__ T st[2 * NN * NN];
+    __shared__ T sp[P];
+    __shared__ T tA[NC][P], tB[NC][P];
+    const unsigned parent = blockIdx.x;
+    const int tid = threadIdx.x, sg = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 2 * NN * NN; i += 64 * NC) st[i] = tab[i];
+    for (int i = tid; i < P; i += 64 * NC)
+        sp[i] = Wp[(items ? (size_t)items[2 * parent + 1] : (size_t)parent * P) + i];
+    __syncthreads();
+    const T *cur = sp;
+    T *bufs[2] = {tA[sg], tB[sg]};
+    int nb = 0;
+    for (int d = D - 1; d >= 1; --d) {
+        const int bit = (sg >> (D - 1 - d)) & 1;
+        apply_dim<T, D, NN>(st + bit * NN * NN, true, d, cur, bufs[nb], lane);
+        __syncthreads();
+        cur = bufs[nb];
+        nb ^= 1;
+    }
+    const T *M = st + ((sg >> (D - 1)) & 1) * NN * NN;
+    constexpr int stride = ipow_c(NN, D - 1);
+    T *dst = Wc + (items ? (size_t)items[2 * parent] : (size_t)parent * NC * P) + (size_t)sg * P;
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const int e = lane + 64 * r;
+        if (e < P) {
+            const int k0 = e / stride, base = e - k0 * stride;
+            T v = T(0);
+#pragma unroll
+            for (int c = 0; c < NN; ++c) v += M[c * NN + k0] * cur[base + c * stride];
+            dst[e] += v;
+        }
     }
 }
 
@@ -606,21 +782,45 @@ __global__ __launch_bounds__(256) void l2p_combine_ct(L2PArgs<T> a, P2PCombineAr
 template <typename T, int D>
 static bool p2m_fast(const P2MArgs<T> &a, long nbox, hipStream_t st)
 {
+    if constexpr (D == 3) {
+        switch (a.nn) {
+        case 4: hipLaunchKernelGGL((p2m3_ct<T, 4>), dim3((unsigned)nbox), dim3(64), 0, st, a); return true;
+        case 5: hipLaunchKernelGGL((p2m3_ct<T, 5>), dim3((unsigned)nbox), dim3(64), 0, st, a); return true;
+        case 6: hipLaunchKernelGGL((p2m3_ct<T, 6>), dim3((unsigned)nbox), dim3(64), 0, st, a); return true;
+        default: break;
+        }
+    }
     FMMA_NN_SWITCH(a.nn, hipLaunchKernelGGL((p2m_ct<T, D, NN>), dim3((unsigned)nbox), dim3(64), 0, st, a));
+}
+
+template <typename T, int D, int NN>
+static void launch_m2m(const T *Wc, T *Wp, long nparent, const T *tab, hipStream_t st, const long *items)
+{
+    if constexpr (kWideTransfer<D, NN>)
+        hipLaunchKernelGGL((m2m_w<T, D, NN>), dim3((unsigned)nparent), dim3(64 << D), 0, st, Wc, Wp, tab, items);
+    else
+        hipLaunchKernelGGL((m2m_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wc, Wp, tab, items);
+}
+
+template <typename T, int D, int NN>
+static void launch_l2l(const T *Wp, T *Wc, long nparent, const T *tab, hipStream_t st, const long *items)
+{
+    if constexpr (kWideTransfer<D, NN>)
+        hipLaunchKernelGGL((l2l_w<T, D, NN>), dim3((unsigned)nparent), dim3(64 << D), 0, st, Wp, Wc, tab, items);
+    else
+        hipLaunchKernelGGL((l2l_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wp, Wc, tab, items);
 }
 
 template <typename T, int D>
 static bool m2m_fast(const T *Wc, T *Wp, long nparent, const T *tab, int nn, hipStream_t st, const long *items)
 {
-    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((m2m_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wc, Wp, tab,
-                                          items));
+    FMMA_NN_SWITCH(nn, (launch_m2m<T, D, NN>(Wc, Wp, nparent, tab, st, items)));
 }
 
 template <typename T, int D>
 static bool l2l_fast(const T *Wp, T *Wc, long nparent, const T *tab, int nn, hipStream_t st, const long *items)
 {
-    FMMA_NN_SWITCH(nn, hipLaunchKernelGGL((l2l_ct<T, D, NN>), dim3((unsigned)nparent), dim3(64), 0, st, Wp, Wc, tab,
-                                          items));
+    FMMA_NN_SWITCH(nn, (launch_l2l<T, D, NN>(Wp, Wc, nparent, tab, st, items)));
 }
 
 template <typename T, int D>
