@@ -1126,24 +1126,29 @@ __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K 
     __shared__ T sx[4][16][D], sy[4][8][D];
     const int b = blockIdx.z, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oi = PAIRED ? a.pairs[2 * blockIdx.y] : (int)blockIdx.y;
-    const long per = (long)a.MP * a.KP;
+    const int per = a.MP * a.KP;  // at most 512 x 512
     const bool bf = a.format == FMMA_FRAG_BF16, wide = a.format == FMMA_FRAG_F32W;
-    const long rw = (long)blockIdx.x * 256 + wave * 64;  // the wave's first element
-    int rbase, cbase, cstep;                             // rows rbase + i, columns cbase + cstep j
+    const int rw = (int)blockIdx.x * 256 + wave * 64;  // the wave's first element
+    // The wave's elements are rows rbase + rs and columns cbase + cstep cs,
+    // (rs, cs) from the lane alone (the fragment order of frag_rowcol).
+    int rbase, cbase, cstep, rs, cs;
     if (wide) {
-        const int l0 = (int)((rw >> 2) & 63);
-        const long q = rw >> 8;
-        const int g = (int)(q % (a.KP / 8)), rb = (int)(q / (a.KP / 8));
+        const int l0 = (rw >> 2) & 63, q = rw >> 8;
+        const int g = q % (a.KP / 8), rb = q / (a.KP / 8);
         rbase = 32 * rb + (l0 & 31);
         cbase = 8 * g + (l0 >> 5);
         cstep = 2;
+        rs = lane >> 2;
+        cs = lane & 3;
     } else {
         const int nks = bf ? a.KP / 32 : a.KP / 4;
-        const long q = bf ? (rw >> 9) : (rw >> 6);
-        const int s = (int)(q % nks), rb = (int)(q / nks);
+        const int q = bf ? (rw >> 9) : (rw >> 6);
+        const int s = q % nks, rb = q / nks;
         rbase = 16 * rb;
-        cbase = bf ? 32 * s + 8 * ((int)((rw >> 3) & 63) >> 4) : 4 * s;
+        cbase = bf ? 32 * s + 8 * (((rw >> 3) & 63) >> 4) : 4 * s;
         cstep = 1;
+        rs = bf ? ((rw >> 3) & 15) + (lane >> 3) : lane & 15;
+        cs = bf ? lane & 7 : lane >> 4;
     }
     const int ncol = bf ? 8 : 4;
     const T Len = static_cast<const T *>(a.Len)[(long)a.batch_problem[b] * a.len_stride];
@@ -1165,13 +1170,12 @@ __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K 
         }
     }
     __syncthreads();
-    const long r = (long)blockIdx.x * 256 + tid;
+    const int r = rw + lane;
     if (r >= per) return;
-    int row, col;
-    frag_rowcol(a.format, a.KP, r, row, col);
+    const int row = rbase + rs, col = cbase + cstep * cs;
     const bool in = row < a.P && col < a.P;
     T v = T(0);
-    if (in) v = kern.template operator()<T, D>(sx[wave][row - rbase], sy[wave][(col - cbase) / cstep]);
+    if (in) v = kern.template operator()<T, D>(sx[wave][rs], sy[wave][cs]);
     auto store = [&](int off, long e, T x) {
         const long g = ((long)b * a.n_off + off) * per + e;
         if (a.format == FMMA_FRAG_F64) {
