@@ -467,11 +467,13 @@ static int build_tree(Plan &pl, const T *pts, const T *w, long n, int L, bool wi
     if (with_w && (rc = wbuf->ensure((size_t)n * sizeof(T)))) return rc;
     if ((rc = pl.pos_alt.ensure((size_t)(D + 1) * n * sizeof(T)))) return rc;
     const Geom<T> *g = pl.geom.as<Geom<T>>();
-    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream))) return rc;
+    if ((rc = leaf_keys<T, D>(pts, n, g, L, grid_n, keys.as<unsigned>(), pl.stream,
+                              bits > 0 ? hist.as<unsigned>() : nullptr)))
+        return rc;
     PointSortBuffers<T> sb{keys.as<unsigned>(), vals.as<unsigned>(), keys_alt.as<unsigned>(),
                            vals_alt.as<unsigned>(), hist.as<unsigned>(), pos.as<T>(),
                            with_w ? wbuf->as<T>() : nullptr, pl.pos_alt.as<T>(), pl.pos_alt.as<T>() + (size_t)D * n};
-    if ((rc = sort_points<T, D>(sb, n, bits, pts, w, with_w, pl.stream))) return rc;
+    if ((rc = sort_points<T, D>(sb, n, bits, pts, w, with_w, pl.stream, bits > 0))) return rc;
     return box_ranges(keys.as<unsigned>(), n, nbox, start.as<unsigned>(), pl.stream);
 }
 

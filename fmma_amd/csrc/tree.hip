@@ -139,47 +139,70 @@ int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_si
 
 // ---------------------------------------------------------------- leaf keys
 
-template <typename T, int D>
-__global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
-                                                   int grid_n, unsigned *keys)
-{
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int NL = grid_n > 0 ? grid_n : 1 << leaf_level;
-    const T len = g->len_leaf;
-    int b[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) b[d] = leaf_coord(pts[i * D + d], g->origin[d], len, NL);
-    if (grid_n > 0) {
-        unsigned id = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) id = id * (unsigned)grid_n + (unsigned)b[d];
-        keys[i] = id;
-    } else {
-        keys[i] = morton_encode<D>(b);
-    }
-}
-
-template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st)
-{
-    if (n <= 0) return FMMA_OK;
-    hipLaunchKernelGGL((keys_kernel<T, D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pts, n, g,
-                       leaf_level, grid_n, keys);
-    FMMA_LAUNCHED("leaf_keys");
-    return FMMA_OK;
-}
-
 // --------------------------------------------------------------- radix sort
 //
 // LSD passes of 8-bit digits over tiles of 4096 keys.  rs_hist counts digits
-// per tile, scan_exclusive turns the digit-major [256][tiles] table into
-// scatter bases, and rs_scatter_points ranks the tile's keys stably and moves
-// every array of the points to its sorted place.
+// per tile (the first pass's counts come from the key kernel), scan_exclusive
+// turns the digit-major [256][tiles] table into scatter bases, and
+// rs_scatter_points ranks the tile's keys stably and moves every array of
+// the points to its sorted place.
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+template <typename T, int D>
+__device__ __forceinline__ unsigned leaf_key(const T *pts, long i, const Geom<T> *g, int NL, bool row_major)
+{
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = leaf_coord(pts[i * D + d], g->origin[d], g->len_leaf, NL);
+    if (row_major) {
+        unsigned id = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) id = id * (unsigned)NL + (unsigned)b[d];
+        return id;
+    }
+    return morton_encode<D>(b);
+}
+
+// One workgroup per sort tile: the keys of its 4096 points and, with hist,
+// the tile's counts of the lowest digit (the first sort pass's histogram).
+template <typename T, int D>
+__global__ __launch_bounds__(256) void keys_kernel(const T *pts, long n, const Geom<T> *g, int leaf_level,
+                                                   int grid_n, unsigned *keys, unsigned *hist, unsigned ntiles)
+{
+    __shared__ unsigned h[256];
+    if (hist) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int NL = grid_n > 0 ? grid_n : 1 << leaf_level;
+    const long tile0 = (long)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long i = tile0 + r * RS_THREADS + threadIdx.x;
+        if (i < n) {
+            const unsigned k = leaf_key<T, D>(pts, i, g, NL, grid_n > 0);
+            keys[i] = k;
+            if (hist) atomicAdd(&h[k & 255u], 1u);
+        }
+    }
+    if (hist) {
+        __syncthreads();
+        hist[(long)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    }
+}
+
+template <typename T, int D>
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st,
+              unsigned *hist0)
+{
+    if (n <= 0) return FMMA_OK;
+    const unsigned ntiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
+    hipLaunchKernelGGL((keys_kernel<T, D>), dim3(ntiles), dim3(256), 0, st, pts, n, g, leaf_level, grid_n, keys,
+                       hist0, ntiles);
+    FMMA_LAUNCHED("leaf_keys");
+    return FMMA_OK;
+}
 
 __global__ __launch_bounds__(256) void rs_hist(const unsigned *keys, long n, int shift, unsigned *hist,
                                                unsigned ntiles)
@@ -405,7 +428,8 @@ __global__ __launch_bounds__(256) void iota_kernel(unsigned *v, long n)
 }
 
 template <typename T, int D>
-int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st)
+int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st,
+                bool hist0_ready)
 {
     if (n <= 0) return FMMA_OK;
     if (bits <= 0) {  // one box: the identity permutation
@@ -424,7 +448,8 @@ int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T 
         const bool to_final = (passes - 1 - p) % 2 == 0;
         T *pout = to_final ? b.pos : b.pos_alt, *wout = to_final ? b.ws : b.ws_alt;
         const int shift = 8 * p;
-        hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
+        if (p > 0 || !hist0_ready)
+            hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(256), 0, st, kin, n, shift, b.hist, ntiles);
         int rc = scan_exclusive(b.hist, (long)ntiles * 256, b.hist + radix_hist_words(n) - SCAN_BLOCKS, st);
         if (rc) return rc;
         PointPass<T> a{kin, vin, kout, vout, pin, win, pout, wout, n, shift, b.hist, ntiles};
@@ -505,9 +530,9 @@ int gather(const T *pts, const T *w, const unsigned *perm, long n, T *pos, T *ws
 
 #define FMMA_TREE_INST(T, D)                                                                                    \
     template int bbox<T, D>(const T *, long, const T *, long, T *, Geom<T> *, int, hipStream_t);                \
-    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t);        \
+    template int leaf_keys<T, D>(const T *, long, const Geom<T> *, int, int, unsigned *, hipStream_t, unsigned *); \
     template int gather<T, D>(const T *, const T *, const unsigned *, long, T *, T *, hipStream_t);              \
-    template int sort_points<T, D>(PointSortBuffers<T> &, long, int, const T *, const T *, bool, hipStream_t);
+    template int sort_points<T, D>(PointSortBuffers<T> &, long, int, const T *, const T *, bool, hipStream_t, bool);
 FMMA_TREE_INST(double, 1)
 FMMA_TREE_INST(double, 2)
 FMMA_TREE_INST(double, 3)

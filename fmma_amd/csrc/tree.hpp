@@ -38,9 +38,11 @@ template <typename T, int D>
 int bbox(const T *t, long nt, const T *s, long ns, T *part, Geom<T> *g, int n_side, hipStream_t st);
 
 // Leaf box of each point: Morton key on the 2^leaf_level grid, or with
-// grid_n > 0 the row-major index on a grid_n^D grid.
+// grid_n > 0 the row-major index on a grid_n^D grid.  With hist0 also the
+// point sort's first-pass histogram (sort_points with hist0_ready).
 template <typename T, int D>
-int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st);
+int leaf_keys(const T *pts, long n, const Geom<T> *g, int leaf_level, int grid_n, unsigned *keys, hipStream_t st,
+              unsigned *hist0 = nullptr);
 
 size_t radix_hist_words(long n);
 
@@ -50,7 +52,8 @@ size_t radix_hist_words(long n);
 // weight when with_w.  The coordinates and weights move with the keys in
 // every pass, so no gather by permutation follows.
 template <typename T, int D>
-int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st);
+int sort_points(PointSortBuffers<T> &b, long n, int bits, const T *pts, const T *w, bool with_w, hipStream_t st,
+                bool hist0_ready = false);
 
 int box_ranges(const unsigned *keys, long n, unsigned nbox, unsigned *start, hipStream_t st);
 
