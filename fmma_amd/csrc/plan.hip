@@ -178,6 +178,8 @@ struct Plan {
     std::vector<std::string> names_out;
     hipEvent_t ev[kStages + 1] = {};
     bool events_ready = false;
+    void *h_geom = nullptr;         // pinned host copy of the bounding cube
+    hipEvent_t geom_ev = nullptr;   // recorded after that copy
 
     DBuf geom, bbox_part;
     DBuf s_keys, s_vals, s_keys_alt, s_vals_alt, s_hist, s_pos, s_w, s_start;
@@ -236,6 +238,8 @@ struct Plan {
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         if (events_ready)
             for (auto &e : ev) (void)hipEventDestroy(e);
+        if (geom_ev) (void)hipEventDestroy(geom_ev);
+        if (h_geom) (void)hipHostFree(h_geom);
     }
 
     size_t tsize() const { return dtype == FMMA_F64 ? 8 : 4; }
@@ -423,6 +427,35 @@ static int check_geom(Plan &pl, Geom<T> *out = nullptr)
     return geom_status_error(g.status);
 }
 
+// The same in two steps: geom_fetch queues the copy of the bounding cube to
+// pinned host memory and returns at once, geom_wait waits for that copy
+// (long done by then) and checks it.  The kernels queued in between (tree,
+// P2M, M2M) write only the plan's own buffers and stay in bounds on any cube
+// (leaf coordinates are clamped), so the solve still fails before anything
+// reaches ans.  Without an allocation the check is made at once
+// (check_geom): the no-allocation entry point allocates no pinned memory.
+template <typename T>
+static int geom_fetch(Plan &pl, Geom<T> *now, bool &pending)
+{
+    pending = false;
+    if (t_arena) return check_geom<T>(pl, now);
+    if (!pl.h_geom) FMMA_HIP(hipHostMalloc(&pl.h_geom, sizeof(Geom<double>), hipHostMallocDefault));
+    if (!pl.geom_ev) FMMA_HIP(hipEventCreateWithFlags(&pl.geom_ev, hipEventDisableTiming));
+    FMMA_HIP(hipMemcpyAsync(pl.h_geom, pl.geom.p, sizeof(Geom<T>), hipMemcpyDeviceToHost, pl.stream));
+    FMMA_HIP(hipEventRecord(pl.geom_ev, pl.stream));
+    pending = true;
+    return FMMA_OK;
+}
+
+template <typename T>
+static int geom_wait(Plan &pl, Geom<T> *out, bool pending)
+{
+    if (!pending) return FMMA_OK;
+    FMMA_HIP(hipEventSynchronize(pl.geom_ev));
+    std::memcpy(out, pl.h_geom, sizeof(Geom<T>));
+    return geom_status_error(out->status);
+}
+
 // With trans_sym_flag 0 the reference evaluates f at absolute node positions
 // len*(b + (1+c)/2) + origin (src/fmm_core.cpp:466-487); per-offset
 // operators use the relative ones of flag 1 (src/fmm_core.cpp:407-408).  For
@@ -571,7 +604,8 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
         return rc;
     if ((rc = bbox<T, D>(tg, nt, sr, ns, pl.bbox_part.as<T>(), pl.geom.as<Geom<T>>(), 1 << L, st))) return rc;
     Geom<T> hgeom;
-    if ((rc = check_geom<T>(pl, &hgeom))) return rc;
+    bool geom_pending;
+    if ((rc = geom_fetch<T>(pl, &hgeom, geom_pending))) return rc;
     if ((rc = build_tree<T, D>(pl, sr, w, ns, L, true, pl.s_keys, pl.s_vals, pl.s_keys_alt, pl.s_vals_alt,
                                pl.s_hist, pl.s_pos, &pl.s_w, pl.s_start)))
         return rc;
@@ -599,6 +633,7 @@ static int fmm_passes(Plan &pl, long nt, const T *tg, const T *w, long ns, const
         if ((rc = m2m<T, D>(Wm + pl.lvl_off[l + 1], Wm + pl.lvl_off[l], 1L << (D * l), pl.d_tab.as<T>(), n, P, st)))
             return rc;
     pl.mark(3);
+    if ((rc = geom_wait<T>(pl, &hgeom, geom_pending))) return rc;
 
     if (L >= 2) {
         const bool sym =
@@ -1111,7 +1146,7 @@ static int run_op(Plan &pl, Op op, long nt, const void *tg, const void *w, long 
     if (!pl.ops.p2p || pl.ops.dim != pl.dim || pl.ops.dtype != pl.dtype)
         return fail(FMMA_ERR_INVALID, "kernel launchers do not match the plan's DIM/TYPE");
     if (pl.timing && !pl.events_ready) {
-        for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
+        for (auto &e : pl.ev) FMMA_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         pl.events_ready = true;
     }
     // the fmm, tree and nrnmm solves record their own stage marks
@@ -1610,7 +1645,7 @@ static int run_batch(Plan &pl, int nprob, const long *tc, const void *tg, const 
     if (!pl.ops.p2p || pl.ops.dim != pl.dim || pl.ops.dtype != pl.dtype)
         return fail(FMMA_ERR_INVALID, "kernel launchers do not match the plan's DIM/TYPE");
     if (pl.timing && !pl.events_ready) {
-        for (auto &e : pl.ev) FMMA_HIP(hipEventCreate(&e));
+        for (auto &e : pl.ev) FMMA_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         pl.events_ready = true;
     }
     pl.view.valid = false;  // the batch reuses the tree buffers of single solves
