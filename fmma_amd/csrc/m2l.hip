@@ -144,7 +144,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
     }
     n_cls -= n3;  // interaction offsets per class
     const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
-    // workgroups per task (row chunks); one for the 32x32x2 form (4 waves x 32 rows)
+    // workgroups per task (row chunks); one for the 32x32 forms (4 waves x 32 rows)
     const long gy = m2l_wide(mode, KP) ? 1 : (MP / 16 + 4 * rb - 1) / (4 * rb);
     // Task order inside a level: tile-major (a tile's 2^D classes together)
     // or class-major (all tiles of one class together, so that each XCD's
@@ -175,7 +175,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         int parts = 1;
         if (splits && split_target > 0 && piped) {
             // in workgroups of 64 rows (one of the 32x32x2 form covers two)
-            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (m2l_wide(mode, KP) ? 2 : gy);
+            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (mode == 1 && m2l_wide(mode, KP) ? 2 : gy);
             long p = split_target / wg;
             if (p > 8) p = 8;
             if (p > n_cls / 16) p = n_cls / 16;
@@ -215,13 +215,15 @@ long m2l_split_elems(const std::vector<M2LSplit> &splits)
     return e;
 }
 
-bool m2l_wide(int mode, int KP) { return mode == 1 && KP == 128; }
+bool m2l_wide(int mode, int KP) { return (mode == 1 || mode == 2) && KP == 128; }
 
 int m2l_frag_format(int P, int mode)
 {
     int MP, KP;
     m2l_shape(P, mode, MP, KP);
-    return mode == 0 ? FMMA_FRAG_F64 : mode == 2 ? FMMA_FRAG_BF16 : m2l_wide(mode, KP) ? FMMA_FRAG_F32W : FMMA_FRAG_F32;
+    const bool wide = m2l_wide(mode, KP);
+    return mode == 0 ? FMMA_FRAG_F64 : mode == 2 ? (wide ? FMMA_FRAG_BF16W : FMMA_FRAG_BF16)
+                                                 : (wide ? FMMA_FRAG_F32W : FMMA_FRAG_F32);
 }
 
 void m2l_shape(int P, int mode, int &MP, int &KP)
@@ -274,14 +276,14 @@ __global__ __launch_bounds__(256) void frag4_kernel(const T *Kraw, CT *Kf, int P
 
 template <typename T>
 __global__ __launch_bounds__(256) void frag_bf16_kernel(const T *Kraw, __bf16 *Kh, __bf16 *Kl, int P, int MP,
-                                                        int KP, long total)
+                                                        int KP, long total, int format)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) return;
     const long per = (long)MP * KP;
     const long f = e / per;
     int row, k;
-    detail::frag_rowcol(FMMA_FRAG_BF16, KP, e - f * per, row, k);
+    detail::frag_rowcol(format, KP, e - f * per, row, k);
     const float v = (row < P && k < P) ? (float)Kraw[(f * P + row) * P + k] : 0.0f;
     const __bf16 h = (__bf16)v;
     Kh[e] = h;
@@ -303,7 +305,7 @@ int m2l_fragments(const T *Kraw, int n_far, int P, int mode, void *Kf, void *Kf_
                            m2l_frag_format(P, mode));
     else
         hipLaunchKernelGGL((frag_bf16_kernel<T>), dim3(nb), dim3(256), 0, st, Kraw, (__bf16 *)Kf, (__bf16 *)Kf_lo, P,
-                           MP, KP, total);
+                           MP, KP, total, m2l_frag_format(P, mode));
     FMMA_LAUNCHED("m2l_fragments");
     return FMMA_OK;
 }
@@ -1002,9 +1004,172 @@ static bool launch_pipe_wide(const M2LKArgs &a, int tb, long ntasks, hipStream_t
     }
 }
 
+// BF16x3 form at a padded length of 128 on the 32x32x16 BF16 MFMA, laid out
+// as m2l_pipe_wide (4 waves x 32 rows, TB boxes, two chunks of 64 per
+// offset, operator and operand loads of step j + 1 during step j).  Per
+// 16-wide k-step and column block the three products lo x hi, hi x lo,
+// hi x hi go into the FP32 accumulator (as in m2l_pipe); an offset's sum is
+// added into FP64 registers.  Operator fragments are FMMA_FRAG_BF16W.
+template <typename TW, int D, int TB>
+__global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
+{
+    constexpr int KP = 128, KC = 64, NCH = KP / KC, NKC = KC / 16;
+    constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 32;
+    constexpr int LDB = KC + kLdsPad<2>;
+    typedef float f16v __attribute__((ext_vector_type(16)));
+    static_assert(EPT % 8 == 0, "gather chunks must be whole 16-byte vectors");
+    constexpr int BUF = 2 * TB * LDB;  // hi tile, then lo tile
+    __shared__ __align__(16) __bf16 Bs[2 * BUF];
+
+    const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
+    const unsigned tix = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const M2LTask task = a.tasks[tix];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int N = 1 << task.level, P = a.P;
+    const int col = tid / TPC, part = tid % TPC;
+
+    int tc[D];
+    const bool cvalid = col < task.np;
+    {
+        int pc[D];
+        morton_decode<D>(task.p0 + (cvalid ? col : 0), pc);
+#pragma unroll
+        for (int d = 0; d < D; ++d) tc[d] = 2 * pc[d] + ((task.cls >> (D - 1 - d)) & 1);
+    }
+    const __bf16 *Wh = static_cast<const __bf16 *>(a.Wc) + task.c_off;
+    const __bf16 *Wlo = static_cast<const __bf16 *>(a.Wc_lo) + task.c_off;
+    const int i0 = task.i1 ? task.i0 : 0, i1 = task.i1 ? task.i1 : a.n_cls;
+    const int *cls_f = a.cls_f + task.cls * a.n_cls + i0;
+    const int *cls_o = a.cls_o + (task.cls * a.n_cls + i0) * D;
+    const bf8_t *Kh = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf) + task.k_off);
+    const bf8_t *Kl = reinterpret_cast<const bf8_t *>(static_cast<const __bf16 *>(a.Kf_lo) + task.k_off);
+
+    uint4 g[EPT / 8], g_lo[EPT / 8];
+    auto gather = [&](int j) {
+        const int i = j / NCH, c = j - i * NCH;
+        int m[D];
+        bool ok = cvalid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m[d] = tc[d] + cls_o[i * D + d];
+            ok = ok && m[d] >= 0 && m[d] < N;
+        }
+        const size_t off = (size_t)(ok ? morton_encode<D>(m) : 0u) * KP + c * KC + part * EPT;
+        const uint4 *sh = reinterpret_cast<const uint4 *>(Wh + off);
+        const uint4 *sl = reinterpret_cast<const uint4 *>(Wlo + off);
+#pragma unroll
+        for (int e = 0; e < EPT / 8; ++e) {
+            g[e] = ok ? sh[e] : uint4{0, 0, 0, 0};
+            g_lo[e] = ok ? sl[e] : uint4{0, 0, 0, 0};
+        }
+    };
+    bf8_t ah[NKC], al[NKC];
+    auto load_a = [&](int j) {
+        const int i = j / NCH, c = j - i * NCH;
+        const size_t base = (((size_t)cls_f[i] * 4 + wave) * (KP / 16) + (size_t)c * NKC) * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < NKC; ++s) {
+            ah[s] = Kh[base + (size_t)s * 64];
+            al[s] = Kl[base + (size_t)s * 64];
+        }
+    };
+    auto store_tile = [&](__bf16 *tile) {
+        __bf16 *dst = tile + col * LDB + part * EPT;
+#pragma unroll
+        for (int e = 0; e < EPT / 8; ++e) {
+            reinterpret_cast<uint4 *>(dst)[e] = g[e];
+            reinterpret_cast<uint4 *>(dst + TB * LDB)[e] = g_lo[e];
+        }
+    };
+
+    f16v acc[NCB];
+    double accd[NCB][16];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            acc[cb][r] = 0.0f;
+            accd[cb][r] = 0.0;
+        }
+    }
+    const int n = (i1 - i0) * NCH;
+    gather(0);
+    load_a(0);
+    store_tile(Bs);
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        const __bf16 *cur = Bs + (i & 1) * BUF;
+        bf8_t ch[NKC], cl[NKC];
+#pragma unroll
+        for (int s = 0; s < NKC; ++s) {
+            ch[s] = ah[s];
+            cl[s] = al[s];
+        }
+        if (i + 1 < n) {
+            gather(i + 1);
+            load_a(i + 1);
+        }
+#pragma unroll
+        for (int s = 0; s < NKC; ++s) {
+            const int kk = 16 * s + 8 * (lane >> 5);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int off = (32 * cb + (lane & 31)) * LDB + kk;
+                const bf8_t bh = *reinterpret_cast<const bf8_t *>(cur + off);
+                const bf8_t bl = *reinterpret_cast<const bf8_t *>(cur + TB * LDB + off);
+                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl[s], bh, acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bl, acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bh, acc[cb], 0, 0, 0);
+            }
+        }
+        if ((i + 1) % NCH == 0) {  // one offset summed: into FP64
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    accd[cb][r] += (double)acc[cb][r];
+                    acc[cb][r] = 0.0f;
+                }
+        }
+        if (i + 1 < n) {
+            store_tile(Bs + ((i + 1) & 1) * BUF);
+            __syncthreads();
+        }
+    }
+    TW *Wl = task.i1 ? static_cast<TW *>(a.Wpart) + task.o_off : static_cast<TW *>(a.Wl) + task.w_off;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+        const int c = 32 * cb + (lane & 31);
+        if (c >= task.np) continue;
+        const size_t box = ((size_t)(task.p0 + c) << D) | (unsigned)task.cls;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 32 * wave + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (row < P) Wl[box * P + row] = (TW)accd[cb][r];
+        }
+    }
+}
+
+template <typename T, int D>
+static bool launch_pipe_bfw(const M2LKArgs &a, int tb, long ntasks, hipStream_t st)
+{
+    if constexpr (D == 1) {
+        return false;
+    } else {
+        const dim3 grid((unsigned)ntasks);
+        if (tb == 32)
+            hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 32>), grid, dim3(256), 0, st, a);
+        else if (tb == 64)
+            hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 64>), grid, dim3(256), 0, st, a);
+        else
+            return false;
+        return true;
+    }
+}
+
 bool m2l_pipe_supported(int D, int KP, int mode, int tb)
 {
-    if (m2l_wide(mode, KP)) return D >= 2 && (tb == 32 || tb == 64);
+    if (m2l_wide(mode, KP)) return D >= 2 && (tb == 32 || tb == 64);  // m2l_pipe_wide, m2l_pipe_bfw
     const int vec = mode == 2 ? 8 : mode == 1 ? 4 : 2;
     const bool small = D >= 2 && (KP == 32 || KP == 64 || KP == 128 || (D == 2 && mode != 2 && (KP == 48 || KP == 96)));
     const bool big = D == 3 && (mode == 2 ? (KP == 256 || KP == 384 || KP == 512)
@@ -1051,6 +1216,11 @@ int m2l_apply(const M2LKArgs &a0, const std::vector<M2LGroup> &groups, int mode,
                 hipLaunchKernelGGL((m2l_mfma4<T, D, 1>), grid, dim3(256), lds, st, a);
             }
         } else {
+            if (m2l_wide(mode, a.KP)) {  // the operators are in the 32x32x16 order: no other kernel reads them
+                if (!g.pipelined || !launch_pipe_bfw<T, D>(a, g.tb, g.count, st)) return FMMA_ERR_UNSUPPORTED;
+                FMMA_LAUNCHED("m2l_apply");
+                continue;
+            }
             done = g.pipelined && launch_pipe<T, D, 2>(a, g.tb, grid, st);
             if (!done) {
                 const size_t lds = (size_t)2 * 64 * (a.KP + 8) * sizeof(__bf16);

@@ -9,6 +9,8 @@
 //       r = ((rb * (KP/32) + s) * 64 + lane) * 8 + j,  row = 16 rb + lane % 16, col = 32 s + 8 (lane / 16) + j
 //   FMMA_FRAG_F32W (32x32x2, four steps per lane side by side for one 16-byte load; MP = 128):
 //       r = ((rb * (KP/8) + g) * 64 + lane) * 4 + j,   row = 32 rb + lane % 32, col = 2 (4 g + j) + lane / 32
+//   FMMA_FRAG_BF16W (32x32x16, hi and lo parts in two arrays; MP = 128):
+//       r = ((rb * (KP/16) + s) * 64 + lane) * 8 + j,  row = 32 rb + lane % 32, col = 16 s + 8 (lane / 32) + j
 #pragma once
 #include "../../fmma_hip.h"
 
@@ -17,7 +19,13 @@ namespace detail {
 
 __host__ __device__ inline void frag_rowcol(int format, int KP, long r, int &row, int &col)
 {
-    if (format == FMMA_FRAG_F32W) {
+    if (format == FMMA_FRAG_BF16W) {
+        const int j = (int)(r & 7), lane = (int)((r >> 3) & 63);
+        const long q = r >> 9;
+        const int s = (int)(q % (KP / 16)), rb = (int)(q / (KP / 16));
+        row = 32 * rb + (lane & 31);
+        col = 16 * s + 8 * (lane >> 5) + j;
+    } else if (format == FMMA_FRAG_F32W) {
         const int j = (int)(r & 3), lane = (int)((r >> 2) & 63);
         const long q = r >> 8;
         const int g = (int)(q % (KP / 8)), rb = (int)(q / (KP / 8));

@@ -1099,6 +1099,10 @@ __global__ __launch_bounds__(256) void m2l_op_kernel(fmma_m2l_op_args a, K kern)
 // Fragment index of element (row, col) of one offset's block.
 __device__ __forceinline__ long frag_index(int format, int KP, int row, int col)
 {
+    if (format == FMMA_FRAG_BF16W) {
+        const int lane = (row & 31) + 32 * ((col & 15) >> 3);
+        return ((((long)(row >> 5) * (KP / 16) + (col >> 4)) * 64 + lane) << 3) + (col & 7);
+    }
     if (format == FMMA_FRAG_F32W) {
         const int lane = (row & 31) + 32 * (col & 1), s = col >> 1;
         return ((((long)(row >> 5) * (KP / 8) + (s >> 2)) * 64 + lane) << 2) + (s & 3);
@@ -1127,12 +1131,21 @@ __global__ __launch_bounds__(256) void m2l_op_frag_kernel(fmma_m2l_op_args a, K 
     const int b = blockIdx.z, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oi = PAIRED ? a.pairs[2 * blockIdx.y] : (int)blockIdx.y;
     const int per = a.MP * a.KP;  // at most 512 x 512
-    const bool bf = a.format == FMMA_FRAG_BF16, wide = a.format == FMMA_FRAG_F32W;
+    const bool bfw = a.format == FMMA_FRAG_BF16W;
+    const bool bf = a.format == FMMA_FRAG_BF16 || bfw, wide = a.format == FMMA_FRAG_F32W;
     const int rw = (int)blockIdx.x * 256 + wave * 64;  // the wave's first element
     // The wave's elements are rows rbase + rs and columns cbase + cstep cs,
     // (rs, cs) from the lane alone (the fragment order of frag_rowcol).
     int rbase, cbase, cstep, rs, cs;
-    if (wide) {
+    if (bfw) {
+        const int l0 = (rw >> 3) & 63, q = rw >> 9;
+        const int st = q % (a.KP / 16), rb = q / (a.KP / 16);
+        rbase = 32 * rb + (l0 & 31);
+        cbase = 16 * st + 8 * (l0 >> 5);
+        cstep = 1;
+        rs = lane >> 3;
+        cs = lane & 7;
+    } else if (wide) {
         const int l0 = (rw >> 2) & 63, q = rw >> 8;
         const int g = q % (a.KP / 8), rb = q / (a.KP / 8);
         rbase = 32 * rb + (l0 & 31);

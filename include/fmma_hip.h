@@ -199,9 +199,17 @@ typedef struct {
 } fmma_exact_grad_args;
 
 /* Element order of M2L operator fragments (include/fmma/detail/m2l_layout.hpp):
- * 16x16x4 MFMA operands (F64, F32), 16x16x32 (BF16), or 32x32x2 FP32 operands
- * in groups of four steps (F32W, MP = KP = 128). */
-enum { FMMA_FRAG_RAW = 0, FMMA_FRAG_F64 = 1, FMMA_FRAG_F32 = 2, FMMA_FRAG_BF16 = 3, FMMA_FRAG_F32W = 4 };
+ * 16x16x4 MFMA operands (F64, F32), 16x16x32 (BF16), 32x32x2 FP32 operands
+ * in groups of four steps (F32W) or 32x32x16 BF16 operands (BF16W), the
+ * last two for MP = KP = 128. */
+enum {
+    FMMA_FRAG_RAW = 0,
+    FMMA_FRAG_F64 = 1,
+    FMMA_FRAG_F32 = 2,
+    FMMA_FRAG_BF16 = 3,
+    FMMA_FRAG_F32W = 4,
+    FMMA_FRAG_BF16W = 5
+};
 
 /* The bounding cube lives on the device (it is reduced there); the box side
  * at a level is len = Len / (TYPE)(1 << level), as in src/fmm_core.cpp:376.
