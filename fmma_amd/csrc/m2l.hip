@@ -174,8 +174,8 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         runs.back().push_back((int)t.size());
         int parts = 1;
         if (splits && split_target > 0 && piped) {
-            // in workgroups of 64 rows (one of the 32x32x2 form covers two)
-            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (mode == 1 && m2l_wide(mode, KP) ? 2 : gy);
+            // in workgroups of 64 rows (one of the 32x32 forms covers two)
+            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (m2l_wide(mode, KP) ? 2 : gy);
             long p = split_target / wg;
             if (p > 8) p = 8;
             if (p > n_cls / 16) p = n_cls / 16;
