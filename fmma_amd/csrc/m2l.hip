@@ -851,15 +851,17 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 // 4 waves covers all 128 rows (wave w: rows 32 w .. 32 w + 31) of TB target
 // boxes, so a partner tile is gathered once per task and every operator
 // value and every staged operand value feeds twice the flops of the 16x16x4
-// shape.  The contraction runs in two chunks of 64 per offset, pipelined as
-// in m2l_pipe (gather and operator loads of step j + 1 during the MFMAs of
-// step j, two LDS tiles); an offset's FP32 sum is added into FP64 registers.
+// shape.  The contraction runs in chunks of KC (64; 32 for 64 boxes, which
+// then fits three waves per SIMD: M2L 2.16 -> 2.13 ms at 5 nodes, N = 4M)
+// per offset, pipelined as in m2l_pipe (gather and operator loads of step
+// j + 1 during the MFMAs of step j, two LDS tiles); an offset's FP32 sum is
+// added into FP64 registers.
 // Operator fragments are FMMA_FRAG_F32W: one 16-byte load per lane brings 4
 // steps.
-template <typename TW, int D, int TB>
-__global__ __launch_bounds__(256) void m2l_pipe_wide(M2LKArgs a)
+template <typename TW, int D, int TB, int KC = 64>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m2l_pipe_wide(M2LKArgs a)
 {
-    constexpr int KP = 128, KC = 64, NCH = KP / KC;
+    constexpr int KP = 128, NCH = KP / KC;
     constexpr int NG = KC / 8;                    // 16-byte operator groups per chunk (4 steps of 2)
     constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 32;
     constexpr int LDB = KC + 1;                   // odd: a 32-lane read of one k hits 32 banks
@@ -997,7 +999,7 @@ static bool launch_pipe_wide(const M2LKArgs &a, int tb, long ntasks, hipStream_t
         if (tb == 32)
             hipLaunchKernelGGL((m2l_pipe_wide<T, D, 32>), grid, dim3(256), 0, st, a);
         else if (tb == 64)
-            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 64>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 64, 32>), grid, dim3(256), 0, st, a);
         else
             return false;
         return true;
