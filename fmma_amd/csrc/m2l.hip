@@ -144,8 +144,8 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
     }
     n_cls -= n3;  // interaction offsets per class
     const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
-    // workgroups per task (row chunks); one for the 32x32 forms (4 waves x 32 rows)
-    const long gy = m2l_wide(mode, KP) ? 1 : (MP / 16 + 4 * rb - 1) / (4 * rb);
+    // workgroups per task (row chunks); 128 rows each for the 32x32 forms (4 waves x 32 rows)
+    const long gy = m2l_wide(mode, KP) ? (MP + 127) / 128 : (MP / 16 + 4 * rb - 1) / (4 * rb);
     // Task order inside a level: tile-major (a tile's 2^D classes together)
     // or class-major (all tiles of one class together, so that each XCD's
     // contiguous share holds few classes and its tasks read the same
@@ -175,7 +175,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         int parts = 1;
         if (splits && split_target > 0 && piped) {
             // in workgroups of 64 rows (one of the 32x32 forms covers two)
-            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (m2l_wide(mode, KP) ? 2 : gy);
+            const long wg = (long)((npar + tb - 1) / tb) * (1L << D) * (m2l_wide(mode, KP) ? 2 * gy : gy);
             long p = split_target / wg;
             if (p > 8) p = 8;
             if (p > n_cls / 16) p = n_cls / 16;
@@ -215,7 +215,7 @@ long m2l_split_elems(const std::vector<M2LSplit> &splits)
     return e;
 }
 
-bool m2l_wide(int mode, int KP) { return (mode == 1 || mode == 2) && KP == 128; }
+bool m2l_wide(int mode, int KP) { return (mode == 1 && (KP == 128 || KP == 224)) || (mode == 2 && KP == 128); }
 
 int m2l_frag_format(int P, int mode)
 {
@@ -249,7 +249,7 @@ void m2l_shape(int P, int mode, int &MP, int &KP)
     } else {
         KP = (P + 31) / 32 * 32;
     }
-    if (m2l_wide(mode, KP)) MP = 128;  // whole 32-row blocks
+    if (m2l_wide(mode, KP)) MP = KP;  // whole 32-row blocks
 }
 
 size_t m2l_frag_elems(int n_far, int P, int mode)
@@ -847,9 +847,10 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
     return false;
 }
 
-// FP32 form at a padded length of 128 on the 32x32x2 MFMA: a workgroup of
-// 4 waves covers all 128 rows (wave w: rows 32 w .. 32 w + 31) of TB target
-// boxes, so a partner tile is gathered once per task and every operator
+// FP32 form at a padded length of 128 or 224 on the 32x32x2 MFMA: a
+// workgroup of 4 waves covers 128 rows (wave w: rows 32 w .. 32 w + 31; at
+// 224, 7 blocks of 32, two workgroups per task) of TB target boxes, so a
+// partner tile is gathered once per 128 rows and every operator
 // value and every staged operand value feeds twice the flops of the 16x16x4
 // shape.  The contraction runs in chunks of KC (64; 32 for 64 boxes, which
 // then fits three waves per SIMD: M2L 2.16 -> 2.13 ms at 5 nodes, N = 4M)
@@ -858,10 +859,11 @@ static bool launch_pipe(const M2LKArgs &a, int tb, dim3 grid, hipStream_t st)
 // added into FP64 registers.
 // Operator fragments are FMMA_FRAG_F32W: one 16-byte load per lane brings 4
 // steps.
-template <typename TW, int D, int TB, int KC = 64>
+template <typename TW, int D, int TB, int KC = 64, int KP = 128>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m2l_pipe_wide(M2LKArgs a)
 {
-    constexpr int KP = 128, NCH = KP / KC;
+    static_assert(KP % KC == 0, "the chunk divides the padded length");
+    constexpr int NCH = KP / KC;
     constexpr int NG = KC / 8;                    // 16-byte operator groups per chunk (4 steps of 2)
     constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 32;
     constexpr int LDB = KC + 1;                   // odd: a 32-lane read of one k hits 32 banks
@@ -870,10 +872,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
     constexpr int BUF = TB * LDB;
     __shared__ __align__(16) float Bs[2 * BUF];
 
+    // row chunks of 128 (KP > 128: several per task) on neighbouring workgroups of one XCD
+    constexpr unsigned GY = (unsigned)(KP + 127) / 128;
     const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
-    const unsigned tix = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const unsigned v = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const unsigned tix = v / GY;
     const M2LTask task = a.tasks[tix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rb = (int)(v - tix * GY) * 4 + wave;  // the wave's 32-row block
+    const bool active = rb * 32 < a.MP;
     const int N = 1 << task.level, P = a.P;
     const int col = tid / TPC, part = tid % TPC;
 
@@ -908,8 +915,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
     };
     float4 afr[NG];
     auto load_a = [&](int j) {
+        if (!active) return;
         const int i = j / NCH, c = j - i * NCH;
-        const size_t base = (((size_t)cls_f[i] * 4 + wave) * (KP / 8) + (size_t)c * NG) * 64 + lane;
+        const size_t base = (((size_t)cls_f[i] * (KP / 32) + rb) * (KP / 8) + (size_t)c * NG) * 64 + lane;
 #pragma unroll
         for (int q = 0; q < NG; ++q) afr[q] = Kf[base + (size_t)q * 64];
     };
@@ -948,16 +956,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
             gather(i + 1);
             load_a(i + 1);
         }
+        if (active) {
 #pragma unroll
-        for (int q = 0; q < NG; ++q) {
+            for (int q = 0; q < NG; ++q) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float av = j == 0 ? acur[q].x : j == 1 ? acur[q].y : j == 2 ? acur[q].z : acur[q].w;
-                const int kk = 2 * (4 * q + j) + (lane >> 5);
+                for (int j = 0; j < 4; ++j) {
+                    const float av = j == 0 ? acur[q].x : j == 1 ? acur[q].y : j == 2 ? acur[q].z : acur[q].w;
+                    const int kk = 2 * (4 * q + j) + (lane >> 5);
 #pragma unroll
-                for (int cb = 0; cb < NCB; ++cb) {
-                    const float bv = cur[(32 * cb + (lane & 31)) * LDB + kk];
-                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[cb], 0, 0, 0);
+                    for (int cb = 0; cb < NCB; ++cb) {
+                        const float bv = cur[(32 * cb + (lane & 31)) * LDB + kk];
+                        acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[cb], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -975,6 +985,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
             __syncthreads();
         }
     }
+    if (!active) return;
     TW *Wl = task.i1 ? static_cast<TW *>(a.Wpart) + task.o_off : static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
@@ -983,7 +994,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
         const size_t box = ((size_t)(task.p0 + c) << D) | (unsigned)task.cls;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = 32 * wave + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             if (row < P) Wl[box * P + row] = (TW)accd[cb][r];
         }
     }
@@ -995,11 +1006,16 @@ static bool launch_pipe_wide(const M2LKArgs &a, int tb, long ntasks, hipStream_t
     if constexpr (D == 1) {
         return false;
     } else {
-        const dim3 grid((unsigned)ntasks);
-        if (tb == 32)
+        const unsigned gy = (unsigned)(a.KP + 127) / 128;
+        const dim3 grid((unsigned)ntasks * gy);
+        if (a.KP == 128 && tb == 32)
             hipLaunchKernelGGL((m2l_pipe_wide<T, D, 32>), grid, dim3(256), 0, st, a);
-        else if (tb == 64)
+        else if (a.KP == 128 && tb == 64)
             hipLaunchKernelGGL((m2l_pipe_wide<T, D, 64, 32>), grid, dim3(256), 0, st, a);
+        else if (a.KP == 224 && tb == 32)
+            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 32, 32, 224>), grid, dim3(256), 0, st, a);
+        else if (a.KP == 224 && tb == 64)
+            hipLaunchKernelGGL((m2l_pipe_wide<T, D, 64, 32, 224>), grid, dim3(256), 0, st, a);
         else
             return false;
         return true;

@@ -259,13 +259,14 @@ def test_m2l_forms_at_512_coefficients(gpu, oracle, prec, tol):
 _TILE_REF = {}
 
 
-@pytest.mark.parametrize("p", [3, 5, 6])
+@pytest.mark.parametrize("p", [3, 4, 5, 6])
 @pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-6)])
 @pytest.mark.parametrize("tile", [16, 32, 64, 128, 256])
 def test_m2l_every_tile_width(gpu, oracle, p, prec, tol, tile):
-    """Every pipelined M2L kernel (tile widths 16..256 target boxes, the
-    chunked contraction for 6 and 7 nodes per dimension) against the oracle,
-    Depth 4 (full and partial tiles)."""
+    """Every pipelined M2L kernel (tile widths 16..256 target boxes; the
+    32x32 FP32 and BF16x3 kernels at 5 nodes per dimension and FP32 at 6;
+    the chunked contraction for 6 and 7 nodes) against the oracle, Depth 4
+    (full and partial tiles)."""
     x = synth.points(95, 4000, 3)
     w = synth.uniform(96, 4000)
     if p not in _TILE_REF:
