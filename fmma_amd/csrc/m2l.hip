@@ -215,7 +215,10 @@ long m2l_split_elems(const std::vector<M2LSplit> &splits)
     return e;
 }
 
-bool m2l_wide(int mode, int KP) { return (mode == 1 && (KP == 128 || KP == 224)) || (mode == 2 && KP == 128); }
+bool m2l_wide(int mode, int KP)
+{
+    return (mode == 1 && (KP == 128 || KP == 224)) || (mode == 2 && (KP == 128 || KP == 256));
+}
 
 int m2l_frag_format(int P, int mode)
 {
@@ -249,7 +252,7 @@ void m2l_shape(int P, int mode, int &MP, int &KP)
     } else {
         KP = (P + 31) / 32 * 32;
     }
-    if (m2l_wide(mode, KP)) MP = KP;  // whole 32-row blocks
+    if (m2l_wide(mode, KP)) MP = (P + 31) / 32 * 32;  // whole 32-row blocks
 }
 
 size_t m2l_frag_elems(int n_far, int P, int mode)
@@ -917,7 +920,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
     auto load_a = [&](int j) {
         if (!active) return;
         const int i = j / NCH, c = j - i * NCH;
-        const size_t base = (((size_t)cls_f[i] * (KP / 32) + rb) * (KP / 8) + (size_t)c * NG) * 64 + lane;
+        const size_t base = (((size_t)cls_f[i] * (a.MP / 32) + rb) * (KP / 8) + (size_t)c * NG) * 64 + lane;
 #pragma unroll
         for (int q = 0; q < NG; ++q) afr[q] = Kf[base + (size_t)q * 64];
     };
@@ -1022,16 +1025,16 @@ static bool launch_pipe_wide(const M2LKArgs &a, int tb, long ntasks, hipStream_t
     }
 }
 
-// BF16x3 form at a padded length of 128 on the 32x32x16 BF16 MFMA, laid out
-// as m2l_pipe_wide (4 waves x 32 rows, TB boxes, two chunks of 64 per
+// BF16x3 form at a padded length of 128 or 256 on the 32x32x16 BF16 MFMA,
+// laid out as m2l_pipe_wide (4 waves x 32 rows, TB boxes, chunks of 64 per
 // offset, operator and operand loads of step j + 1 during step j).  Per
 // 16-wide k-step and column block the three products lo x hi, hi x lo,
 // hi x hi go into the FP32 accumulator (as in m2l_pipe); an offset's sum is
 // added into FP64 registers.  Operator fragments are FMMA_FRAG_BF16W.
-template <typename TW, int D, int TB>
+template <typename TW, int D, int TB, int KP = 128>
 __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
 {
-    constexpr int KP = 128, KC = 64, NCH = KP / KC, NKC = KC / 16;
+    constexpr int KC = 64, NCH = KP / KC, NKC = KC / 16;
     constexpr int TPC = 256 / TB, EPT = KC / TPC, NCB = TB / 32;
     constexpr int LDB = KC + kLdsPad<2>;
     typedef float f16v __attribute__((ext_vector_type(16)));
@@ -1039,10 +1042,15 @@ __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
     constexpr int BUF = 2 * TB * LDB;  // hi tile, then lo tile
     __shared__ __align__(16) __bf16 Bs[2 * BUF];
 
+    // row chunks of 128 (more than 128 rows: several per task) on neighbouring workgroups of one XCD
+    const unsigned gy = (unsigned)(a.MP + 127) / 128;
     const unsigned n8 = gridDim.x / 8u, rem = gridDim.x % 8u, xcd = blockIdx.x % 8u;
-    const unsigned tix = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const unsigned v = xcd * n8 + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const unsigned tix = v / gy;
     const M2LTask task = a.tasks[tix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rb = (int)(v - tix * gy) * 4 + wave;  // the wave's 32-row block
+    const bool active = rb * 32 < a.MP;
     const int N = 1 << task.level, P = a.P;
     const int col = tid / TPC, part = tid % TPC;
 
@@ -1083,8 +1091,9 @@ __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
     };
     bf8_t ah[NKC], al[NKC];
     auto load_a = [&](int j) {
+        if (!active) return;
         const int i = j / NCH, c = j - i * NCH;
-        const size_t base = (((size_t)cls_f[i] * 4 + wave) * (KP / 16) + (size_t)c * NKC) * 64 + lane;
+        const size_t base = (((size_t)cls_f[i] * (a.MP / 32) + rb) * (KP / 16) + (size_t)c * NKC) * 64 + lane;
 #pragma unroll
         for (int s = 0; s < NKC; ++s) {
             ah[s] = Kh[base + (size_t)s * 64];
@@ -1127,17 +1136,19 @@ __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
             gather(i + 1);
             load_a(i + 1);
         }
+        if (active) {
 #pragma unroll
-        for (int s = 0; s < NKC; ++s) {
-            const int kk = 16 * s + 8 * (lane >> 5);
+            for (int s = 0; s < NKC; ++s) {
+                const int kk = 16 * s + 8 * (lane >> 5);
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const int off = (32 * cb + (lane & 31)) * LDB + kk;
-                const bf8_t bh = *reinterpret_cast<const bf8_t *>(cur + off);
-                const bf8_t bl = *reinterpret_cast<const bf8_t *>(cur + TB * LDB + off);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl[s], bh, acc[cb], 0, 0, 0);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bl, acc[cb], 0, 0, 0);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bh, acc[cb], 0, 0, 0);
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const int off = (32 * cb + (lane & 31)) * LDB + kk;
+                    const bf8_t bh = *reinterpret_cast<const bf8_t *>(cur + off);
+                    const bf8_t bl = *reinterpret_cast<const bf8_t *>(cur + TB * LDB + off);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl[s], bh, acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bl, acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch[s], bh, acc[cb], 0, 0, 0);
+                }
             }
         }
         if ((i + 1) % NCH == 0) {  // one offset summed: into FP64
@@ -1154,6 +1165,7 @@ __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
             __syncthreads();
         }
     }
+    if (!active) return;
     TW *Wl = task.i1 ? static_cast<TW *>(a.Wpart) + task.o_off : static_cast<TW *>(a.Wl) + task.w_off;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
@@ -1162,7 +1174,7 @@ __global__ __launch_bounds__(256) void m2l_pipe_bfw(M2LKArgs a)
         const size_t box = ((size_t)(task.p0 + c) << D) | (unsigned)task.cls;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = 32 * wave + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             if (row < P) Wl[box * P + row] = (TW)accd[cb][r];
         }
     }
@@ -1174,11 +1186,15 @@ static bool launch_pipe_bfw(const M2LKArgs &a, int tb, long ntasks, hipStream_t 
     if constexpr (D == 1) {
         return false;
     } else {
-        const dim3 grid((unsigned)ntasks);
-        if (tb == 32)
+        const dim3 grid((unsigned)ntasks * (unsigned)((a.MP + 127) / 128));
+        if (a.KP == 128 && tb == 32)
             hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 32>), grid, dim3(256), 0, st, a);
-        else if (tb == 64)
+        else if (a.KP == 128 && tb == 64)
             hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 64>), grid, dim3(256), 0, st, a);
+        else if (a.KP == 256 && tb == 32)
+            hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 32, 256>), grid, dim3(256), 0, st, a);
+        else if (a.KP == 256 && tb == 64)
+            hipLaunchKernelGGL((m2l_pipe_bfw<T, D, 64, 256>), grid, dim3(256), 0, st, a);
         else
             return false;
         return true;
