@@ -93,7 +93,7 @@ constexpr int m2l_chunk(int KP, int mode)
 // Row blocks of 16 per wave: 2 for BF16x3 at a padded length of 128 (one
 // workgroup covers all MP <= 128 rows; measured 1.15 -> 1.10 ms at 5 nodes,
 // N = 4M), else 1 (FP32 at 2 measured 2.7 -> 3.8 ms, FP64 unchanged).
-constexpr int m2l_row_blocks(int KP, int mode) { return KP == 128 && mode == 2 ? 2 : 1; }
+constexpr int m2l_row_blocks(int KP, int mode) { return (KP == 128 && mode == 2) || (KP == 224 && mode == 0) ? 2 : 1; }
 
 // Reorders the tasks [first, first + count) of one pipelined launch, made of
 // runs of unequal cost (levels; a split level's tasks are shorter), so that
@@ -143,7 +143,7 @@ std::vector<M2LTask> make_m2l_tasks(int D, int leaf_level, int P, int mode, std:
         n3 *= 3;
     }
     n_cls -= n3;  // interaction offsets per class
-    const int rb = (KP == 128 && mode == 2) ? 2 : 1;  // m2l_row_blocks
+    const int rb = m2l_row_blocks(KP, mode);
     // workgroups per task (row chunks); 128 rows each for the 32x32 forms (4 waves x 32 rows)
     const long gy = m2l_wide(mode, KP) ? (MP + 127) / 128 : (MP / 16 + 4 * rb - 1) / (4 * rb);
     // Task order inside a level: tile-major (a tile's 2^D classes together)
