@@ -309,13 +309,16 @@ def test_m2l_full_tiles_depth5(gpu, oracle, prec, tol, tile):
 _2D_REF = {}
 
 
-@pytest.mark.parametrize("p,depth,n", [(5, 5, 12000), (5, 6, 40000), (3, 5, 12000), (7, 5, 12000)])
+@pytest.mark.parametrize("p,depth,n", [(5, 5, 12000), (5, 6, 40000), (3, 5, 12000), (7, 5, 12000), (10, 5, 12000),
+                                       (14, 4, 8000)])
 @pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-6), ("bf16x3", 1e-5)])
 def test_m2l_forms_2d_log(gpu, oracle, p, depth, n, prec, tol):
     """The 2-D precision forms (config 2's kernel, log|x-y| with 0 at r = 0,
     targets = sources, weights N(0,1)): 6 nodes per dimension (P = 36, padded
     to 48 / 64 for the pipelined kernels), 4 nodes (P = 16: the non-pipelined
-    kernels) and 8 nodes (P = 64), against the oracle.
+    kernels), 8 nodes (P = 64), 11 nodes (P = 121: the 32x32 FP32 and BF16x3
+    kernels at a padded length of 128) and 15 nodes (P = 225: BF16x3 at 256),
+    against the oracle.
 
     BF16x3's bar is 1e-5: its two-term BF16 splits carry about 17 significant
     bits per operand (FP32: 24), and each log|x-y| operator is dominated by
