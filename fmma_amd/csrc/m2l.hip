@@ -80,14 +80,15 @@ void OffsetTables::build(int dim)
 }
 
 // The contraction chunk of a padded length: all of it below 128 (and at 128
-// for FP32 and BF16x3); FP64 at 128 in chunks of 64 (the smaller operand
-// tiles and fragments raise occupancy: 6.0 -> 5.0 ms at 5 nodes, N = 4M);
+// for FP32 and BF16x3); FP64 at 128 in chunks of 32 (the smaller operand
+// tiles and fragments in flight: 6.0 ms in one chunk, 4.6 ms in 64s, 4.4 ms
+// in 32s at 5 nodes, N = 4M);
 // beyond 128, 32 (FP64, FP32: KP is then a multiple of 32) or 128 (BF16x3: a
 // multiple of 128), so that a step's MFMA work still covers the next step's
 // loads.
 constexpr int m2l_chunk(int KP, int mode)
 {
-    return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 64 : 128) : 32;
+    return KP < 128 ? KP : mode == 2 ? 128 : KP == 128 ? (mode == 0 ? 32 : 128) : 32;
 }
 
 // Row blocks of 16 per wave: 2 for BF16x3 at a padded length of 128 (one
